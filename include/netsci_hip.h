@@ -1,0 +1,138 @@
+/*
+ * netsci_hip.h — the C-ABI of libnsx.so, the MI355X (gfx950) engine behind
+ * the netcalc / cuarray API.
+ *
+ * Conventions:
+ *   - Every function returns 0 on success and a non-zero status otherwise;
+ *     nsx_last_error() returns a thread-local message for the last failure.
+ *     No C++ exceptions cross this boundary.
+ *   - Pointers named d_* are device (HBM) pointers, h_* are host pointers.
+ *   - `stream` is a hipStream_t passed as void*; NULL is the default stream.
+ *     Entry points that take a stream are stream-ordered and return before
+ *     the work finishes unless stated otherwise.
+ *   - X holds one node per row. Within a row, coordinate dimension di of
+ *     frame f sits at X[row*ld + di*dstride + f] — the reference's layout
+ *     (netchem/src/network.cpp:65-69, mutual_information_cpu.cpp:27-45
+ *     indexing `i + di*n`), so for a plain call dstride == n.
+ */
+#ifndef NETSCI_HIP_H
+#define NETSCI_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status and device ---------------------------------------------- */
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *nsx_last_error(void);
+
+/* Number of visible HIP devices (0 without a GPU; never fails). */
+int nsx_device_count(int *count);
+
+/* Make `device` current for the calling thread and check it is gfx950. */
+int nsx_init(int device);
+
+/* Device name and CU count, for run records. name gets at most len bytes. */
+int nsx_device_info(char *name, int len, int *cu_count, int *clock_khz);
+
+/* ---- buffers: the HIP allocator behind CuArray --------------------------
+ * Replaces cudaMalloc / cudaFree / cudaMemcpy in CuArray<T>::allocateDevice,
+ * deallocateDevice, toDevice and toHost (cuarray/src/cuarray.cu:104-164).
+ * nsx_memcpy_h2d / d2h copy synchronously through a pinned staging ring,
+ * so pageable host buffers transfer at pinned speed. */
+int nsx_alloc(void **d_ptr, size_t bytes);
+int nsx_free(void *d_ptr);
+int nsx_host_alloc(void **h_ptr, size_t bytes);   /* pinned host memory */
+int nsx_host_free(void *h_ptr);
+int nsx_memcpy_h2d(void *d_dst, const void *h_src, size_t bytes);
+int nsx_memcpy_d2h(void *h_dst, const void *d_src, size_t bytes);
+int nsx_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
+int nsx_stream_sync(void *stream);
+int nsx_device_sync(void);
+
+/* Device timing on `stream` (hipEvent pairs). */
+int nsx_event_create(void **ev);
+int nsx_event_destroy(void *ev);
+int nsx_event_record(void *ev, void *stream);
+int nsx_event_elapsed_ms(void *start, void *stop, float *ms);
+
+/* ---- KSG mutual information over node pairs ---------------------------
+ * Replaces the per-pair GPU path of the pair loops
+ * netcalc/src/mutual_information.cpp:103-172 and
+ * netcalc/src/generalized_correlation.cpp:104-173, i.e. one
+ * mutualInformationGpu call (netcalc/src/mutual_information_gpu.cu:1007-1079)
+ * per ab row, with a single persistent launch for all P pairs.
+ *
+ * d_mi_out[p] = KSG MI of rows (d_ab[2p], d_ab[2p+1]) over frames 0..n-1,
+ * equal bit-for-bit to mutualInformationCpu
+ * (netcalc/src/mutual_information_cpu.cpp:6-121), ties at the k-th
+ * neighbour included. 1 <= d <= 3, 1 <= k <= 16, k < n.
+ * d_counts_out (nullable) receives n_x, n_y per frame: [p][f][2] int32. */
+int nsx_ksg_mi(const float *d_X, long long ld, long long dstride, int d,
+               int n, const int *d_ab, long long P, int k, float *d_mi_out,
+               int *d_counts_out, void *stream);
+
+/* Windowed (bootstrap-replicate) variant: replicate r uses frames
+ * [h_win_off[r], h_win_off[r] + h_win_n[r]) of every row;
+ * d_mi_out[r*P + p]. All R*P (replicate, pair) items run in one persistent
+ * launch, longest windows first. No counterpart in the reference, whose
+ * bootstrap resamples and calls the pair loop once per replicate
+ * (docs/tutorial3_bootstrapping.rst:125); each item equals that call on the
+ * window's frames. */
+int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride,
+                       int d, const int *h_win_off, const int *h_win_n, int R,
+                       const int *d_ab, long long P, int k, float *d_mi_out,
+                       void *stream);
+
+/* Frames whose k-th / (k+1)-th joint distances tied in the last KSG launch
+ * on this thread (resolved on the device in libstdc++ std::sort order). */
+int nsx_ksg_last_tie_frames(long long *count);
+
+/* GC transform of generalized_correlation_{cpu,gpu}.cpp:8-33 on host data:
+ * h_gc[p] = MI <= 0 ? 0 : (float)sqrt(1 - (float)exp(-(2/(float)d)*MI)),
+ * evaluated in double like the reference. */
+int nsx_gc_from_mi_host(const float *h_mi, long long P, int d, float *h_gc);
+
+/* psi table of psi.cpp:6-19: h_psi[0..n]. */
+int nsx_psi_host(float *h_psi, int n);
+
+/* ---- host (CPU platform) implementations -------------------------------
+ * The CPU side of the `platform` flag (platform.h:9-10). Same results as the
+ * GPU path; threads over pairs. */
+int nsx_cpu_ksg_mi(const float *h_X, long long ld, long long dstride, int d,
+                   int n, const int *h_ab, long long P, int k,
+                   float *h_mi_out, int threads);
+
+/* libstdc++-order selection used for tie resolution: h_idx[0..m) receives
+ * the first m entries of std::sort(iota(n), by h_key with <). */
+int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx);
+
+/* ---- Hedetniemi all-pairs shortest paths ------------------------------
+ * Replaces hedetniemiAllShortestPathsGpu (netcalc/src/hedetniemi_gpu.cu:148-235,
+ * kernels :47-107). H_{p+1}[i][j] = min_k fl(A[j][k] + H_p[k][i]) for
+ * p = 0..L-1 from H_0 = A, double-buffered (Jacobi), d_Hi[p] = H_p
+ * (L*N*N floats, nullable when d_paths is null), then the backtrack of
+ * :70-107 into d_paths[i][j][0..L-1] (N*L*N ints). Diagonal: paths[i][i][*] = i. */
+int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H,
+                   float *d_Hi, int *d_paths, void *stream);
+
+/* hedetniemiAllShortestPathLengthsGpu (hedetniemi_gpu.cu:109-146): H only. */
+int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H,
+                           void *stream);
+
+/* correlationToAdjacencyGpu (hedetniemi_gpu.cu:237-261, kernel :8-26). */
+int nsx_corr_to_adj(const float *d_C, int N, float *d_A, void *stream);
+
+/* CPU platform versions of the three above (same results). */
+int nsx_cpu_hedetniemi(const float *h_A, int N, int L, float tol, float *h_H,
+                       int *h_paths, int threads);
+int nsx_cpu_corr_to_adj(const float *h_C, int N, float *h_A);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NETSCI_HIP_H */

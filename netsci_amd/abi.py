@@ -1,0 +1,211 @@
+"""ctypes access to libnsx.so's C ABI (include/netsci_hip.h).
+
+This is the binding a Python caller of the C ABI would write (the netcalc /
+cuarray modules go through the C++ host layer instead). It holds device
+buffers as plain pointers and raises RuntimeError with nsx_last_error() on
+any non-zero status — there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnsx.so")
+
+# (name, argtypes) for every exported function, in header order.
+_VP, _I, _LL, _F, _SZ = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_float, ctypes.c_size_t
+_IP, _LLP, _FP = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_float)
+SIGNATURES = {
+    "nsx_last_error": [],
+    "nsx_device_count": [_IP],
+    "nsx_init": [_I],
+    "nsx_device_info": [ctypes.c_char_p, _I, _IP, _IP],
+    "nsx_alloc": [ctypes.POINTER(_VP), _SZ],
+    "nsx_free": [_VP],
+    "nsx_host_alloc": [ctypes.POINTER(_VP), _SZ],
+    "nsx_host_free": [_VP],
+    "nsx_memcpy_h2d": [_VP, _VP, _SZ],
+    "nsx_memcpy_d2h": [_VP, _VP, _SZ],
+    "nsx_memcpy_d2d": [_VP, _VP, _SZ, _VP],
+    "nsx_stream_sync": [_VP],
+    "nsx_device_sync": [],
+    "nsx_event_create": [ctypes.POINTER(_VP)],
+    "nsx_event_destroy": [_VP],
+    "nsx_event_record": [_VP, _VP],
+    "nsx_event_elapsed_ms": [_VP, _VP, _FP],
+    "nsx_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP, _VP],
+    "nsx_ksg_mi_windows": [_VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I, _VP, _VP],
+    "nsx_ksg_last_tie_frames": [_LLP],
+    "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
+    "nsx_psi_host": [_VP, _I],
+    "nsx_cpu_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _I],
+    "nsx_select_prefix_host": [_VP, _I, _I, _VP],
+    "nsx_hedetniemi": [_VP, _I, _I, _F, _VP, _VP, _VP, _VP],
+    "nsx_hedetniemi_lengths": [_VP, _I, _I, _VP, _VP],
+    "nsx_corr_to_adj": [_VP, _I, _VP, _VP],
+    "nsx_cpu_hedetniemi": [_VP, _I, _I, _F, _VP, _VP, _I],
+    "nsx_cpu_corr_to_adj": [_VP, _I, _VP],
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m netsci_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "nsx_last_error" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().nsx_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what}: {msg}" if what else msg)
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    lib().nsx_device_count(ctypes.byref(c))
+    return c.value
+
+
+def init(device=0):
+    check(lib().nsx_init(device), "nsx_init")
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu, clk = ctypes.c_int(0), ctypes.c_int(0)
+    check(lib().nsx_device_info(name, 256, ctypes.byref(cu), ctypes.byref(clk)), "device info")
+    return name.value.decode(), cu.value, clk.value
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class DeviceArray:
+    """A device buffer with a fixed dtype and shape (freed on close/del)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().nsx_alloc(ctypes.byref(p), max(self.nbytes, 4)), "nsx_alloc")
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            check(lib().nsx_memcpy_h2d(d.ptr, _ptr(a), a.nbytes), "upload")
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().nsx_memcpy_d2h(_ptr(out), self.ptr, self.nbytes), "download")
+        return out
+
+    def close(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().nsx_free(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ksg_mi(dX, ld, dstride, d, n, dab, P, k, dmi, dcounts=None, stream=None):
+    check(lib().nsx_ksg_mi(dX.ptr, ld, dstride, d, n, dab.ptr, P, k, dmi.ptr,
+                           dcounts.ptr if dcounts is not None else None, stream), "nsx_ksg_mi")
+
+
+def ksg_mi_windows(dX, ld, dstride, d, windows, dab, P, k, dmi, stream=None):
+    w = np.ascontiguousarray(windows, np.int32).reshape(-1, 2)
+    off = np.ascontiguousarray(w[:, 0])
+    ln = np.ascontiguousarray(w[:, 1])
+    check(lib().nsx_ksg_mi_windows(dX.ptr, ld, dstride, d,
+                                   off.ctypes.data_as(_IP), ln.ctypes.data_as(_IP), len(w),
+                                   dab.ptr, P, k, dmi.ptr, stream), "nsx_ksg_mi_windows")
+
+
+def last_tie_frames():
+    c = ctypes.c_longlong(0)
+    lib().nsx_ksg_last_tie_frames(ctypes.byref(c))
+    return c.value
+
+
+def gc_from_mi(mi, d):
+    mi = np.ascontiguousarray(mi, np.float32)
+    out = np.empty_like(mi)
+    check(lib().nsx_gc_from_mi_host(_ptr(mi), mi.size, d, _ptr(out)), "gc transform")
+    return out
+
+
+def select_prefix(key, m):
+    key = np.ascontiguousarray(key, np.float32)
+    out = np.empty(m, np.int32)
+    check(lib().nsx_select_prefix_host(_ptr(key), key.size, m, _ptr(out)), "select prefix")
+    return out
+
+
+def cpu_ksg_mi(X, ab, k, n, d, threads=0, dstride=None):
+    X = np.ascontiguousarray(X, np.float32)
+    ab = np.ascontiguousarray(ab, np.int32).reshape(-1, 2)
+    out = np.empty(len(ab), np.float32)
+    check(lib().nsx_cpu_ksg_mi(_ptr(X), X.shape[1], dstride or n, d, n, _ptr(ab), len(ab), k,
+                               _ptr(out), threads), "cpu ksg")
+    return out
+
+
+def hedetniemi(dA, N, L, tol, dH, dHi=None, dpaths=None, stream=None):
+    check(lib().nsx_hedetniemi(dA.ptr, N, L, tol, dH.ptr, dHi.ptr if dHi is not None else None,
+                               dpaths.ptr if dpaths is not None else None, stream),
+          "nsx_hedetniemi")
+
+
+def hedetniemi_lengths(dA, N, L, dH, stream=None):
+    check(lib().nsx_hedetniemi_lengths(dA.ptr, N, L, dH.ptr, stream), "nsx_hedetniemi_lengths")
+
+
+def corr_to_adj(dC, N, dA, stream=None):
+    check(lib().nsx_corr_to_adj(dC.ptr, N, dA.ptr, stream), "nsx_corr_to_adj")
+
+
+def sync():
+    check(lib().nsx_device_sync(), "device sync")
+
+
+class Timer:
+    """hipEvent pair on a stream: elapsed device time in ms."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        self.a, self.b = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().nsx_event_create(ctypes.byref(self.a)), "event")
+        check(lib().nsx_event_create(ctypes.byref(self.b)), "event")
+
+    def start(self):
+        check(lib().nsx_event_record(self.a, self.stream), "record")
+
+    def stop(self):
+        check(lib().nsx_event_record(self.b, self.stream), "record")
+
+    def ms(self):
+        v = ctypes.c_float(0)
+        check(lib().nsx_event_elapsed_ms(self.a, self.b, ctypes.byref(v)), "elapsed")
+        return v.value

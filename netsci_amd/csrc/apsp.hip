@@ -1,0 +1,291 @@
+// Hedetniemi all-pairs shortest paths on gfx950.
+//
+// Reference: netcalc/src/hedetniemi_gpu.cu:28-107 (kernels) and :109-261
+// (drivers). One iteration is
+//     H_{p+1}[i][j] = min_k fl(A[j][k] + H_p[k][i])
+// — a min-plus product of H_p^T and A^T. The reference updates H in place
+// while other blocks read it (a race); this engine double-buffers (Jacobi),
+// so every run gives the same H and the same snapshots Hi[p] = H_p.
+//
+// min-plus kernel: 64 x 64 output tile per 256-thread workgroup, 4 x 4
+// outputs per thread, k streamed through LDS in steps of 16 (both operand
+// tiles stored k-major so each thread reads four consecutive values with one
+// ds_read_b128). Each output folds k in ascending order. When A holds no
+// negative zero, v_min_f32 gives exactly the reference's strict-`<` result
+// (sums can only be -0 if both addends are -0); otherwise the kernel takes a
+// compare-and-select loop that keeps the first minimum, as the reference
+// does. Out-of-range rows/columns are padded with +inf, which never wins.
+// Roofline: FP32 VALU, 2 lane-ops (add, min) per (i, j, k).
+//
+// Backtrack kernel (hedetniemi_gpu.cu:70-107): a wavefront owns one (i, j)
+// at a time; for each p from L-1 down to 0 its 64 lanes test 64 candidate q
+// at once and a ballot picks the first q != j with
+// fl(fl(Hi[p][i][q] + A[q][k]) - len) < tol, the reference's first-hit rule.
+// A[q][k] is read from a transposed copy so each probe is coalesced.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/netsci_hip.h"
+#include "nsx_internal.h"
+
+namespace nsx {
+namespace {
+
+constexpr int kTile = 64;   // output tile edge
+constexpr int kKStep = 16;  // k values per LDS stage
+constexpr int kMicro = 4;   // outputs per thread per dimension
+
+__global__ void negzero_kernel(const float *__restrict__ A, long long count,
+                               unsigned *__restrict__ flag) {
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    bool found = false;
+    for (; e < count; e += stride) found |= (__float_as_uint(A[e]) == 0x80000000u);
+    if (__ballot(found) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+// out[i][j] = min_k fl(A[j][k] + Hp[k][i]).
+template <bool kExactTies>
+__device__ __forceinline__ void minplus_tile(const float *__restrict__ A,
+                                             const float *__restrict__ Hp,
+                                             float *__restrict__ out, int N, float (*Hs)[kTile],
+                                             float (*As)[kTile]) {
+    const int tid = threadIdx.x;
+    const int tx = tid & 15;   // i micro-tile
+    const int ty = tid >> 4;   // j micro-tile
+    const int i0 = blockIdx.x * kTile;
+    const int j0 = blockIdx.y * kTile;
+    float acc[kMicro][kMicro];
+#pragma unroll
+    for (int a = 0; a < kMicro; ++a)
+#pragma unroll
+        for (int b = 0; b < kMicro; ++b) acc[a][b] = __builtin_inff();
+    const float inf = __builtin_inff();
+
+    for (int k0 = 0; k0 < N; k0 += kKStep) {
+        // Hs[kk][ii] = Hp[k0+kk][i0+ii]; 16 x 64 values, 4 per thread.
+#pragma unroll
+        for (int r = 0; r < (kKStep * kTile) / 256; ++r) {
+            const int e = r * 256 + tid;
+            const int kk = e / kTile, ii = e % kTile;
+            const int k = k0 + kk, i = i0 + ii;
+            Hs[kk][ii] = (k < N && i < N) ? Hp[(long long)k * N + i] : inf;
+        }
+        // As[kk][jj] = A[j0+jj][k0+kk]; read along k, store k-major.
+#pragma unroll
+        for (int r = 0; r < (kKStep * kTile) / 256; ++r) {
+            const int e = r * 256 + tid;
+            const int jj = e / kKStep, kk = e % kKStep;
+            const int k = k0 + kk, j = j0 + jj;
+            As[kk][jj] = (k < N && j < N) ? A[(long long)j * N + k] : inf;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kKStep; ++kk) {
+            const float4 hv = *reinterpret_cast<const float4 *>(&Hs[kk][tx * kMicro]);
+            const float4 av = *reinterpret_cast<const float4 *>(&As[kk][ty * kMicro]);
+            const float h[4] = {hv.x, hv.y, hv.z, hv.w};
+            const float v[4] = {av.x, av.y, av.z, av.w};
+#pragma unroll
+            for (int a = 0; a < kMicro; ++a)
+#pragma unroll
+                for (int b = 0; b < kMicro; ++b) {
+                    const float s = v[b] + h[a];
+                    if (kExactTies) acc[a][b] = (s < acc[a][b]) ? s : acc[a][b];
+                    else acc[a][b] = __builtin_fminf(acc[a][b], s);
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < kMicro; ++a) {
+        const int i = i0 + tx * kMicro + a;
+        if (i >= N) continue;
+#pragma unroll
+        for (int b = 0; b < kMicro; ++b) {
+            const int j = j0 + ty * kMicro + b;
+            if (j < N) out[(long long)i * N + j] = acc[a][b];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ A,
+                                                      const float *__restrict__ Hp,
+                                                      float *__restrict__ out, int N,
+                                                      const unsigned *__restrict__ negzero) {
+    __shared__ __attribute__((aligned(16))) float Hs[kKStep][kTile];
+    __shared__ __attribute__((aligned(16))) float As[kKStep][kTile];
+    if (*negzero) minplus_tile<true>(A, Hp, out, N, Hs, As);
+    else minplus_tile<false>(A, Hp, out, N, Hs, As);
+}
+
+__global__ void transpose_kernel(const float *__restrict__ in, float *__restrict__ out, int N) {
+    __shared__ float t[32][33];
+    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const int row = by + r, col = bx + threadIdx.x;
+        if (row < N && col < N) t[r][threadIdx.x] = in[(long long)row * N + col];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const int row = bx + r, col = by + threadIdx.x;
+        if (row < N && col < N) out[(long long)row * N + col] = t[threadIdx.x][r];
+    }
+}
+
+// hedetniemi_gpu.cu:70-107 with the diagonal written as paths[i][i][*] = i.
+__global__ __launch_bounds__(256) void backtrack_kernel(const float *__restrict__ H,
+                                                        const float *__restrict__ Hi,
+                                                        const float *__restrict__ AT,
+                                                        int *__restrict__ paths, float tol,
+                                                        int N, int L) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int i = blockIdx.x;
+    const long long NN = (long long)N * N;
+    for (int j = wave; j < N; j += 4) {
+        int *pp = paths + (long long)i * N * L + (long long)j * L;
+        if (i == j) {
+            for (int p = lane; p < L; p += 64) pp[p] = i;
+            continue;
+        }
+        float len = H[(long long)i * N + j];
+        int k = j;
+        for (int p = L - 1; p >= 0; --p) {
+            const float *hrow = Hi + (long long)p * NN + (long long)i * N;
+            const float *arow = AT + (long long)k * N;  // arow[q] = A[q][k]
+            int found = -1;
+            for (int q0 = 0; q0 < N; q0 += 64) {
+                const int q = q0 + lane;
+                bool hit = false;
+                if (q < N && q != j) {
+                    const float t = hrow[q] + arow[q];
+                    hit = (t - len) < tol;
+                }
+                const unsigned long long m = __ballot(hit);
+                if (m) {
+                    found = q0 + __ffsll((long long)m) - 1;
+                    break;
+                }
+            }
+            if (found >= 0) {
+                len -= arow[found];
+                k = found;
+            }
+            if (lane == 0) pp[p] = k;
+        }
+    }
+}
+
+__global__ void corr_to_adj_kernel(const float *__restrict__ C, float *__restrict__ A, int N) {
+    const long long count = (long long)N * N;
+    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (; e < count; e += stride) {
+        const int i = (int)(e / N), j = (int)(e % N);
+        const float c = C[e];
+        A[e] = (i == j) ? 0.0f : (c == 0.0f ? __builtin_inff() : 1.0f / c);
+    }
+}
+
+int launch_minplus(const float *A, const float *Hp, float *out, int N, const unsigned *flag,
+                   hipStream_t s) {
+    const int tiles = (N + kTile - 1) / kTile;
+    hipLaunchKernelGGL(minplus_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, Hp, out, N, flag);
+    NSX_HIP(hipGetLastError(), "min-plus launch");
+    return kOk;
+}
+
+int negzero_flag(const float *A, int N, unsigned **flag, hipStream_t s) {
+    unsigned *f = (unsigned *)workspace(kWsApsp1, 256);
+    if (!f) return kHipError;
+    int rc = fill_u32(f, 0u, 1, s);
+    if (rc) return rc;
+    const long long count = (long long)N * N;
+    const int blocks = (int)std::min<long long>((count + 255) / 256, 2048);
+    hipLaunchKernelGGL(negzero_kernel, dim3(blocks), dim3(256), 0, s, A, count, f);
+    NSX_HIP(hipGetLastError(), "negative-zero scan launch");
+    *flag = f;
+    return kOk;
+}
+
+}  // namespace
+}  // namespace nsx
+
+using namespace nsx;
+
+extern "C" {
+
+int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *stream) {
+    clear_error();
+    NSX_REQUIRE(d_A && d_H && N >= 1 && L >= 0, "nsx_hedetniemi_lengths: invalid arguments");
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = (size_t)N * N * sizeof(float);
+    unsigned *flag = nullptr;
+    rc = negzero_flag(d_A, N, &flag, s);
+    if (rc) return rc;
+    float *tmp = (float *)workspace(kWsApsp0, bytes);
+    if (!tmp) return kHipError;
+    // Ping-pong so that the last iteration lands in d_H.
+    float *bufs[2] = {d_H, tmp};
+    int cur = (L % 2 == 0) ? 0 : 1;
+    NSX_HIP(hipMemcpyAsync(bufs[cur], d_A, bytes, hipMemcpyDeviceToDevice, s), "H0 = A");
+    for (int p = 0; p < L; ++p) {
+        rc = launch_minplus(d_A, bufs[cur], bufs[cur ^ 1], N, flag, s);
+        if (rc) return rc;
+        cur ^= 1;
+    }
+    return kOk;
+}
+
+int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float *d_Hi,
+                   int *d_paths, void *stream) {
+    clear_error();
+    NSX_REQUIRE(d_A && d_H && N >= 1 && L >= 1, "nsx_hedetniemi: invalid arguments");
+    NSX_REQUIRE(!d_paths || d_Hi, "nsx_hedetniemi: paths need the Hi snapshot buffer");
+    if (!d_Hi) return nsx_hedetniemi_lengths(d_A, N, L, d_H, stream);
+    int rc = ensure_device();
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const long long NN = (long long)N * N;
+    const size_t bytes = (size_t)NN * sizeof(float);
+    unsigned *flag = nullptr;
+    rc = negzero_flag(d_A, N, &flag, s);
+    if (rc) return rc;
+    NSX_HIP(hipMemcpyAsync(d_Hi, d_A, bytes, hipMemcpyDeviceToDevice, s), "Hi[0] = A");
+    for (int p = 0; p < L; ++p) {
+        float *out = (p + 1 < L) ? d_Hi + (size_t)(p + 1) * NN : d_H;
+        rc = launch_minplus(d_A, d_Hi + (size_t)p * NN, out, N, flag, s);
+        if (rc) return rc;
+    }
+    if (!d_paths) return kOk;
+    float *AT = (float *)workspace(kWsApsp0, bytes);
+    if (!AT) return kHipError;
+    const int tb = (N + 31) / 32;
+    hipLaunchKernelGGL(transpose_kernel, dim3(tb, tb), dim3(32, 8), 0, s, d_A, AT, N);
+    NSX_HIP(hipGetLastError(), "transpose launch");
+    hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, d_paths, tol,
+                       N, L);
+    NSX_HIP(hipGetLastError(), "backtrack launch");
+    return kOk;
+}
+
+int nsx_corr_to_adj(const float *d_C, int N, float *d_A, void *stream) {
+    clear_error();
+    NSX_REQUIRE(d_C && d_A && N >= 0, "nsx_corr_to_adj: invalid arguments");
+    if (N == 0) return kOk;
+    int rc = ensure_device();
+    if (rc) return rc;
+    const long long count = (long long)N * N;
+    const int blocks = (int)std::min<long long>((count + 255) / 256, 8192);
+    hipLaunchKernelGGL(corr_to_adj_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, d_C,
+                       d_A, N);
+    NSX_HIP(hipGetLastError(), "adjacency launch");
+    return kOk;
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// Internal helpers shared by the libnsx translation units.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include <hip/hip_runtime.h>
+
+namespace nsx {
+
+// Thread-local last-error message behind nsx_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+
+// Return codes.
+enum Status : int {
+    kOk = 0,
+    kInvalidArgument = 1,
+    kHipError = 2,
+    kNoDevice = 3,
+    kInternal = 4,
+};
+
+// Check a HIP call; on failure record "<what>: <hip message>" and return.
+#define NSX_HIP(call, what)                                                    \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            ::nsx::set_error(std::string(what) + ": " + hipGetErrorString(e_)); \
+            return ::nsx::kHipError;                                           \
+        }                                                                      \
+    } while (0)
+
+#define NSX_REQUIRE(cond, msg)                                                 \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            ::nsx::set_error(msg);                                             \
+            return ::nsx::kInvalidArgument;                                    \
+        }                                                                      \
+    } while (0)
+
+// Ensure a usable device is current (initialises once per thread).
+int ensure_device();
+
+// Number of CUs of the current device.
+int cu_count();
+
+// Grow-only device workspace owned by the library, one per slot name.
+// Returns nullptr (and sets the error) on allocation failure.
+void *workspace(int slot, size_t bytes);
+
+enum WorkspaceSlot : int {
+    kWsPsi = 0,
+    kWsCounter = 1,
+    kWsTieScratch = 2,
+    kWsStats = 3,
+    kWsWindows = 4,
+    kWsApsp0 = 5,
+    kWsApsp1 = 6,
+    kWsScreen0 = 7,
+    kWsScreen1 = 8,
+    kWsCount = 16,
+};
+
+// Fill `count` 32-bit words at d_ptr with `value` on `stream` (a kernel, so
+// it may be recorded into a HIP graph).
+int fill_u32(void *d_ptr, uint32_t value, size_t count, hipStream_t stream);
+
+// psi table of psi.cpp:6-19, host side.
+void psi_table(float *psi, int n);
+
+}  // namespace nsx

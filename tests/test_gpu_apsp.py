@@ -1,0 +1,86 @@
+"""Parity of the gfx950 Hedetniemi path with the oracle (GPU required).
+
+Distances, snapshots and backtracked paths must be bit-exact against the
+oracle's Jacobi restatement of hedetniemi_gpu.cu:47-107.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from netsci_amd import abi, cuarray, netcalc
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(N, density, seed, negzero=False):
+    rng = np.random.default_rng(seed)
+    A = rng.uniform(0.1, 3.0, size=(N, N)).astype(np.float32)
+    A[rng.random((N, N)) > density] = np.inf
+    np.fill_diagonal(A, 0.0)
+    if negzero:
+        A[rng.random((N, N)) < 0.02] = -0.0
+    return A
+
+
+def _gpu(A, L, tol):
+    N = A.shape[0]
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    dP = abi.DeviceArray((N * L * N,), np.int32)
+    abi.hedetniemi(dA, N, L, tol, dH, dHi, dP)
+    return dH.numpy(), dHi.numpy(), dP.numpy()
+
+
+@pytest.mark.parametrize("N,L,density,negzero", [(37, 6, 0.15, False), (100, 12, 0.05, False),
+                                                 (130, 5, 0.3, True), (64, 1, 0.5, False)])
+def test_hedetniemi_bitexact(N, L, density, negzero):
+    print(f"apsp N={N} L={L}", flush=True)
+    A = _graph(N, density, seed=N + L, negzero=negzero)
+    H, Hi, P = _gpu(A, L, 1e-4)
+    Ho, Hio, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
+    np.testing.assert_array_equal(H.view(np.uint32), Ho.view(np.uint32))
+    np.testing.assert_array_equal(Hi.view(np.uint32), Hio.view(np.uint32))
+    np.testing.assert_array_equal(P, Po)
+
+
+def test_lengths_only_matches():
+    N, L = 150, 9
+    print("apsp lengths", flush=True)
+    A = _graph(N, 0.04, seed=3)
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    abi.hedetniemi_lengths(dA, N, L, dH)
+    Ho, _, _ = oracle.hedetniemi(A, L, 0.0, want_paths=False)
+    np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
+
+
+def test_netcalc_gpu_equals_cpu_platform_and_paths_decode():
+    N, L = 60, 8
+    print("apsp netcalc", flush=True)
+    A = _graph(N, 0.08, seed=7)
+    Ac = cuarray.FloatCuArray()
+    Ac.fromNumpy2D(A)
+    out = {}
+    for plat in (netcalc.GPU_PLATFORM, netcalc.CPU_PLATFORM):
+        H, P = cuarray.FloatCuArray(), cuarray.IntCuArray()
+        netcalc.hedetniemiAllShortestPaths(A=Ac, H=H, paths=P, maxPathLength=L, tolerance=1e-4,
+                                           platform=plat)
+        out[plat] = (H.toNumpy2D(), P.toNumpy1D(), P)
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    for i, j in [(0, 5), (3, 41), (10, 10)]:
+        got = netcalc.recoverSingleShortestPath(paths=out[0][2], maxPathLength=L, i=i, j=j)
+        np.testing.assert_array_equal(got, oracle.recover_path(out[0][1], N, L, i, j))
+
+
+def test_corr_to_adj_gpu():
+    N = 70
+    print("adjacency", flush=True)
+    rng = np.random.default_rng(1)
+    C = rng.uniform(0, 1, (N, N)).astype(np.float32)
+    C[rng.random((N, N)) < 0.2] = 0.0
+    Cc, A = cuarray.FloatCuArray(), cuarray.FloatCuArray()
+    Cc.fromNumpy2D(C)
+    netcalc.correlationToAdjacency(A=A, C=Cc, n=N, platform=netcalc.GPU_PLATFORM)
+    np.testing.assert_array_equal(A.toNumpy2D(), oracle.corr_to_adj(C))
