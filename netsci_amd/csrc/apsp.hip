@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void backtrack_kernel(const float *__restrict_
                                                         int *__restrict__ paths, float tol,
                                                         int N, int L) {
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = blockIdx.x;
     const long long NN = (long long)N * N;
     for (int j = wave; j < N; j += 4) {
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void backtrack_kernel(const float *__restrict_
                 }
                 const unsigned long long m = __ballot(hit);
                 if (m) {
-                    found = q0 + __ffsll((long long)m) - 1;
+                    found = __builtin_amdgcn_readfirstlane(q0 + __ffsll((long long)m) - 1);
                     break;
                 }
             }

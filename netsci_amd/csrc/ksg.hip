@@ -8,8 +8,9 @@
 // epilogue summed sequentially in frame order.
 //
 // Mapping (one persistent launch for every (window, pair) item):
-//   * a wavefront owns one item at a time, pulled from a global counter, and
-//     walks its query frames in blocks of 64 (lane = query frame);
+//   * a wavefront owns one item at a time (items are dealt round-robin to
+//     the resident waves, longest windows first) and walks its query frames
+//     in blocks of 64 (lane = query frame);
 //   * the item's frames stream through a wave-private LDS chunk (SoA per
 //     coordinate, NaN-padded) and are read back as 16-byte broadcasts, so
 //     every lane compares its query against the same frame j;
@@ -60,7 +61,6 @@ struct KsgArgs {
     float inv_k;         // (float)(1.0 / (float)k)
     float *mi_out;       // [rep][P]
     int *counts_out;     // nullable: [p][frame][2]
-    unsigned long long *counter;
     KeyIdx *scratch;     // per wave: scratch_stride records
     long long scratch_stride;
     unsigned long long *stats;  // [0] tie frames
@@ -209,13 +209,13 @@ __device__ __forceinline__ void pass2_chunk(const Chunk<D> &ch, int cnt4,
 // std::sort order, then take eps over positions 1..k
 // (mutual_information_cpu.cpp:48-77).
 template <int D>
-__device__ void resolve_ties(bool tie, int i, const float (&qa)[D], const float (&qb)[D],
-                             const float *xa, const float *xb, long long ds, int n, int k,
-                             KeyIdx *scratch, int lane, float &ex, float &ey,
-                             unsigned long long *stats) {
+__device__ unsigned resolve_ties(bool tie, const float (&qa)[D], const float (&qb)[D],
+                                 const float *xa, const float *xb, long long ds, int n, int k,
+                                 KeyIdx *scratch, int lane, float &ex, float &ey) {
     unsigned long long tmask = __ballot(tie);
+    const unsigned count = (unsigned)__popcll(tmask);
     while (tmask) {
-        const int t = __ffsll((long long)tmask) - 1;
+        const int t = __builtin_amdgcn_readfirstlane(__ffsll((long long)tmask) - 1);
         float tqa[D], tqb[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -243,38 +243,39 @@ __device__ void resolve_ties(bool tie, int i, const float (&qa)[D], const float 
                 rea = fmaxf(rea, marg_global<D>(tqa, xa, ds, j));
                 reb = fmaxf(reb, marg_global<D>(tqb, xb, ds, j));
             }
-            atomicAdd(stats, 1ull);
         }
         rea = __shfl(rea, 0);
         reb = __shfl(reb, 0);
-        if (lane == t) {
-            ex = rea;
-            ey = reb;
-        }
+        ex = (lane == t) ? rea : ex;
+        ey = (lane == t) ? reb : ey;
         wave_sync_global();
         tmask &= tmask - 1;
     }
-    (void)i;
+    return count;
 }
 
+// Every wave walks items w = wave_gid, wave_gid + nwaves, ... All loop
+// bounds and item fields are wave-uniform scalars; the only lane-divergent
+// regions are the heap insertion, the tie rebuild's single-lane selection
+// and the per-lane loads/stores.
 template <int D, int K>
 __global__ __launch_bounds__(kWave *kWavesPerBlock) void ksg_kernel(KsgArgs a) {
     __shared__ __attribute__((aligned(16))) float lds[kWavesPerBlock][2 * D * kChunk];
+    __shared__ float sterm[kWavesPerBlock][kWave];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wib = threadIdx.x / kWave;
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     Chunk<D> ch{lds[wib]};
+    float *terms = sterm[wib];
+    const long long nwaves = (long long)gridDim.x * kWavesPerBlock;
     const long long wave_gid = (long long)blockIdx.x * kWavesPerBlock + wib;
     KeyIdx *scratch = a.scratch + wave_gid * a.scratch_stride;
-    const unsigned long long total = (unsigned long long)a.R * (unsigned long long)a.P;
+    const long long total = (long long)a.R * a.P;
     const long long ds = a.dstride;
+    unsigned long long ties = 0;
 
-    while (true) {
-        unsigned long long w = 0;
-        if (lane == 0) w = atomicAdd(a.counter, 1ull);
-        w = uniform_u64(w);
-        if (w >= total) break;
-        const int slot = (int)(w / (unsigned long long)a.P);
-        const long long p = (long long)(w - (unsigned long long)slot * a.P);
+    for (long long w = wave_gid; w < total; w += nwaves) {
+        const int slot = (int)(w / a.P);
+        const long long p = w - (long long)slot * a.P;
         const int off = a.win_off[slot];
         const int n = a.win_n[slot];
         const float *xa = a.X + (long long)a.ab[2 * p] * a.ld + off;
@@ -310,15 +311,12 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void ksg_kernel(KsgArgs a) {
             // eps over the k nearest; boundary ties take the exact path.
             const bool tie = valid && (h[K - 1] == h[K]);
             float ex = 0.0f, ey = 0.0f;
-            if (!tie) {
 #pragma unroll
-                for (int m = 0; m < K; ++m) {
-                    ex = fmaxf(ex, marg_global<D>(qa, xa, ds, id[m]));
-                    ey = fmaxf(ey, marg_global<D>(qb, xb, ds, id[m]));
-                }
+            for (int m = 0; m < K; ++m) {
+                ex = fmaxf(ex, marg_global<D>(qa, xa, ds, id[m]));
+                ey = fmaxf(ey, marg_global<D>(qb, xb, ds, id[m]));
             }
-            resolve_ties<D>(tie, i, qa, qb, xa, xb, ds, n, K, scratch, lane, ex, ey,
-                            a.stats);
+            ties += resolve_ties<D>(tie, qa, qb, xa, xb, ds, n, K, scratch, lane, ex, ey);
             // Pass 2: marginal counts.
             int cx = 0, cy = 0;
             for (int c0 = 0; c0 < n; c0 += kChunk) {
@@ -328,23 +326,23 @@ __global__ __launch_bounds__(kWave *kWavesPerBlock) void ksg_kernel(KsgArgs a) {
                 wave_sync();
             }
             const int nx = cx - 1, ny = cy - 1;
-            const float term = valid ? a.psi[nx] + a.psi[ny] : 0.0f;
+            terms[lane] = valid ? a.psi[nx] + a.psi[ny] : 0.0f;
             if (a.counts_out && valid) {
                 a.counts_out[(p * n + i) * 2 + 0] = nx;
                 a.counts_out[(p * n + i) * 2 + 1] = ny;
             }
-            // Sequential float32 sum in frame order.
+            wave_sync();
+            // Sequential float32 sum in frame order (every lane runs the same
+            // sum over LDS broadcasts).
             const int lim = min(kWave, n - q0);
-            for (int l = 0; l < lim; ++l) {
-                acc += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(term), l));
-            }
+            for (int l = 0; l < lim; ++l) acc += terms[l];
+            wave_sync();
         }
-        if (lane == 0) {
-            const float avg = acc / (float)n;
-            const float mi = a.psi[n] + a.psi[a.k] - a.inv_k - avg;
-            a.mi_out[(long long)a.win_rep[slot] * a.P + p] = mi;
-        }
+        const float avg = acc / (float)n;
+        const float mi = a.psi[n] + a.psi[a.k] - a.inv_k - avg;
+        if (lane == 0) a.mi_out[(long long)a.win_rep[slot] * a.P + p] = mi;
     }
+    if (lane == 0 && ties) atomicAdd(a.stats, ties);
 }
 
 using KernelFn = void (*)(KsgArgs);
@@ -437,7 +435,7 @@ int run_ksg(const float *X, long long ld, long long dstride, int d,
                            hipMemcpyHostToDevice, stream), "psi upload");
     NSX_HIP(hipMemcpyAsync(d_win, host_win.data(), host_win.size() * sizeof(int),
                            hipMemcpyHostToDevice, stream), "window upload");
-    rc = fill_u32(d_ctr, 0u, 16, stream);  // counter [0..1], ties [2..3]
+    rc = fill_u32(d_ctr, 0u, 16, stream);  // [0..1]: tie-frame count
     if (rc) return rc;
 
     KsgArgs args;
@@ -455,16 +453,15 @@ int run_ksg(const float *X, long long ld, long long dstride, int d,
     args.inv_k = (float)(1.0 / (float)k);
     args.mi_out = mi_out;
     args.counts_out = counts_out;
-    args.counter = d_ctr;
     args.scratch = d_scr;
     args.scratch_stride = nmax;
-    args.stats = d_ctr + 1;
+    args.stats = d_ctr;
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(kWave * kWavesPerBlock), 0,
                        stream, args);
     NSX_HIP(hipGetLastError(), "ksg launch");
     // Tie statistics for the run record (synchronises the stream).
     unsigned long long ties = 0;
-    NSX_HIP(hipMemcpyAsync(&ties, d_ctr + 1, sizeof(ties), hipMemcpyDeviceToHost, stream),
+    NSX_HIP(hipMemcpyAsync(&ties, d_ctr, sizeof(ties), hipMemcpyDeviceToHost, stream),
             "tie count readback");
     NSX_HIP(hipStreamSynchronize(stream), "ksg sync");
     g_last_ties = (long long)ties;
