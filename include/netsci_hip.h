@@ -107,8 +107,12 @@ int nsx_cpu_ksg_mi(const float *h_X, long long ld, long long dstride, int d,
                    float *h_mi_out, int threads);
 
 /* libstdc++-order selection used for tie resolution: h_idx[0..m) receives
- * the first m entries of std::sort(iota(n), by h_key with <). */
-int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx);
+ * the first m entries of std::sort(iota(n), by h_key with <), as
+ * mutual_information_cpu.cpp:48-53 orders a frame's neighbours.
+ * h_heap_ranges (nullable) gets the number of sub-ranges that took
+ * introsort's heap-sort fallback. */
+int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx,
+                           int *h_heap_ranges);
 
 /* ---- Hedetniemi all-pairs shortest paths ------------------------------
  * Replaces hedetniemiAllShortestPathsGpu (netcalc/src/hedetniemi_gpu.cu:148-235,

@@ -40,7 +40,7 @@ SIGNATURES = {
     "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
     "nsx_psi_host": [_VP, _I],
     "nsx_cpu_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _I],
-    "nsx_select_prefix_host": [_VP, _I, _I, _VP],
+    "nsx_select_prefix_host": [_VP, _I, _I, _VP, _IP],
     "nsx_hedetniemi": [_VP, _I, _I, _F, _VP, _VP, _VP, _VP],
     "nsx_hedetniemi_lengths": [_VP, _I, _I, _VP, _VP],
     "nsx_corr_to_adj": [_VP, _I, _VP, _VP],
@@ -156,11 +156,13 @@ def gc_from_mi(mi, d):
     return out
 
 
-def select_prefix(key, m):
+def select_prefix(key, m, with_heap_count=False):
     key = np.ascontiguousarray(key, np.float32)
     out = np.empty(m, np.int32)
-    check(lib().nsx_select_prefix_host(_ptr(key), key.size, m, _ptr(out)), "select prefix")
-    return out
+    heaps = ctypes.c_int(0)
+    check(lib().nsx_select_prefix_host(_ptr(key), key.size, m, _ptr(out), ctypes.byref(heaps)),
+          "select prefix")
+    return (out, heaps.value) if with_heap_count else out
 
 
 def cpu_ksg_mi(X, ab, k, n, d, threads=0, dstride=None):

@@ -111,7 +111,7 @@ int nsx_cpu_ksg_mi(const float *h_X, long long ld, long long dstride, int d, int
     return 0;
 }
 
-int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx) {
+int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx, int *h_heap_ranges) {
     nsx::clear_error();
     if (!h_key || !h_idx || n < 0 || m < 0 || m > n) {
         nsx::set_error("nsx_select_prefix_host: invalid arguments");
@@ -119,8 +119,10 @@ int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx) {
     }
     std::vector<nsx::KeyIdx> row(n);
     for (int j = 0; j < n; ++j) row[j] = nsx::KeyIdx{h_key[j], j};
-    nsx::sort_prefix(row.data(), n, m);
+    int heaps = 0;
+    nsx::sort_prefix(row.data(), n, m, &heaps);
     for (int j = 0; j < m; ++j) h_idx[j] = row[j].idx;
+    if (h_heap_ranges) *h_heap_ranges = heaps;
     return 0;
 }
 

@@ -134,9 +134,10 @@ NSX_HD inline void insertion_sort(KeyIdx *a, long l) {
 
 // Arrange a[0..n) so that a[0..m) equals what std::sort(a, a + n, kless)
 // puts there. Records beyond the returned bound are left partially ordered.
-// Returns the bound B (m <= B <= n) up to which the prefix is final.
+// Returns the bound B (m <= B <= n) up to which the prefix is final;
+// heap_ranges (nullable) counts sub-ranges that took the heap-sort fallback.
 // Needs no memory besides `a`; the explicit stack holds pending sub-ranges.
-NSX_HD inline long sort_prefix(KeyIdx *a, long n, long m) {
+NSX_HD inline long sort_prefix(KeyIdx *a, long n, long m, int *heap_ranges = nullptr) {
     if (n <= 0) return 0;
     if (m > n) m = n;
     long bound = n;  // smallest partition cut >= m seen so far
@@ -156,6 +157,7 @@ NSX_HD inline long sort_prefix(KeyIdx *a, long n, long m) {
         while (l - f > kThreshold) {
             if (depth == 0) {
                 heap_sort_range(a + f, l - f);
+                if (heap_ranges) ++*heap_ranges;
                 break;
             }
             --depth;

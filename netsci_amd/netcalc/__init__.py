@@ -92,7 +92,9 @@ class MutualInformationSettings(_SettingsMapping):
 @dataclass
 class GeneralizedCorrelationSettings(_SettingsMapping):
     """netcalc/__init__.py:74-128, including the empty-ab auto-fill of all
-    N^2 ordered pairs (i, j), self-pairs included (:90-102)."""
+    N^2 ordered pairs (i, j), self-pairs included (:90-102). The reference
+    takes N = X.n() / d, which is the frame count of a node-row X, and so
+    writes node indices past the last row; here N = X.m(), the node count."""
     X: FloatCuArray
     R: FloatCuArray
     ab: IntCuArray
@@ -110,7 +112,7 @@ class GeneralizedCorrelationSettings(_SettingsMapping):
         if not self.ab.size():
             print("Initializing ab")
             import numpy as np
-            num = int(self.X.n() / self.d)
+            num = int(self.X.m())
             ii, jj = np.meshgrid(np.arange(num, dtype=np.int32),
                                  np.arange(num, dtype=np.int32), indexing="ij")
             self.ab.fromNumpy2D(np.stack([ii.ravel(), jj.ravel()], axis=1))

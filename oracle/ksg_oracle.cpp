@@ -219,4 +219,29 @@ void orc_heapsort_like_libstdcxx(const float *key, int n, int *out) {
                       [&](int p, int q) { return key[p] < key[q]; });
 }
 
+// McIlroy's adversary ("A Killer Adversary for Quicksort", Software:
+// Practice and Experience 29(4), 1999), run against libstdc++'s std::sort:
+// values are fixed lazily as comparisons reveal them, which yields an input
+// that drives the introsort to its depth limit and heap-sort fallback.
+void orc_antiqsort(int n, float *out) {
+    std::vector<int> val(n), ptr(n);
+    const int gas = n - 1;
+    int nsolid = 0, candidate = 0;
+    for (int i = 0; i < n; ++i) {
+        ptr[i] = i;
+        val[i] = gas;
+    }
+    auto cmp = [&](int x, int y) {
+        if (val[x] == gas && val[y] == gas) {
+            if (x == candidate) val[x] = nsolid++;
+            else val[y] = nsolid++;
+        }
+        if (val[x] == gas) candidate = x;
+        else if (val[y] == gas) candidate = y;
+        return val[x] < val[y];
+    };
+    std::sort(ptr.begin(), ptr.end(), cmp);
+    for (int i = 0; i < n; ++i) out[i] = (float)val[i];
+}
+
 }  // extern "C"

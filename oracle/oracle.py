@@ -48,6 +48,7 @@ def lib():
         L.orc_recover_path.restype = ctypes.c_int
         L.orc_argsort_like_reference.argtypes = [_f32p, ctypes.c_int, _i32p]
         L.orc_heapsort_like_libstdcxx.argtypes = [_f32p, ctypes.c_int, _i32p]
+        L.orc_antiqsort.argtypes = [ctypes.c_int, _f32p]
         _lib = L
     return _lib
 
@@ -138,4 +139,11 @@ def heapsort_like_libstdcxx(key):
     key = _f32(key)
     out = np.zeros(key.size, np.int32)
     lib().orc_heapsort_like_libstdcxx(key, key.size, out)
+    return out
+
+
+def antiqsort(n):
+    """An input that drives std::sort into its heap-sort fallback."""
+    out = np.zeros(n, np.float32)
+    lib().orc_antiqsort(n, out)
     return out
