@@ -7,7 +7,7 @@ implementation; both give the reference CPU path's values. Keyword names are
 the reference's (callers pass them by name, e.g.
 docs/tutorial2_protein_structure_gc.rst:89-98).
 """
-from dataclasses import asdict, dataclass
+from dataclasses import dataclass, fields
 from pathlib import Path
 from typing import ItemsView, KeysView, Mapping, Union, ValuesView
 
@@ -42,12 +42,13 @@ class _SettingsMapping(Mapping):
     arguments, without the checkpoint pair when checkpointFrequency == -1."""
 
     def __iter__(self):
+        names = [f.name for f in fields(self)]
         if self.checkpointFrequency == -1:
-            for key in asdict(self):
+            for key in names:
                 if key not in ("checkpointFrequency", "checkpointFileName"):
                     yield key
         else:
-            yield from asdict(self)
+            yield from names
 
     def items(self) -> ItemsView:
         return ItemsView({k: self[k] for k in self})
