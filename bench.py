@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--k", type=int, default=6)
     ap.add_argument("--cpu-pairs", type=int, default=24,
                     help="pairs timed for the CPU baseline (0 to skip)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
 
 
@@ -69,6 +72,14 @@ def main():
     from netsci_amd import abi, synth
 
     abi.init(0)
+    if args.copy_calibration_mb:
+        nbytes = args.copy_calibration_mb << 20
+        src = abi.DeviceArray((nbytes // 4,), np.float32)
+        dst = abi.DeviceArray((nbytes // 4,), np.float32)
+        abi.check(abi.lib().nsx_memcpy_d2d(dst.ptr, src.ptr, nbytes, None), "copy")
+        abi.sync()
+        print(json.dumps({"copy_bytes": nbytes}), flush=True)
+        return
     dev_name, cus, _ = abi.device_info()
     N, n, d, k = args.nodes, args.frames, 3, args.k
     X, _ = synth.calpha_trajectory(N=N, n=n, seed=1)
