@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--k", type=int, default=6)
     ap.add_argument("--cpu-pairs", type=int, default=24,
                     help="pairs timed for the CPU baseline (0 to skip)")
+    ap.add_argument("--workload", default="gc", choices=["gc", "bootstrap", "apsp"],
+                    help="gc: the headline all-pairs GC pass (default); bootstrap: 64 uneven "
+                         "frame windows of a 512-node, 4000-frame trajectory, one graph-captured "
+                         "batched launch vs one launch per replicate; apsp: Hedetniemi on the "
+                         "-log(GC) contact graph of the gc workload")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -80,6 +85,10 @@ def main():
         abi.sync()
         print(json.dumps({"copy_bytes": nbytes}), flush=True)
         return
+    if args.workload == "bootstrap":
+        return bench_bootstrap(args, abi, synth)
+    if args.workload == "apsp":
+        return bench_apsp(args, abi, synth)
     dev_name, cus, _ = abi.device_info()
     N, n, d, k = args.nodes, args.frames, 3, args.k
     X, _ = synth.calpha_trajectory(N=N, n=n, seed=1)
@@ -161,6 +170,119 @@ def main():
         "cpu_baseline": cpu,
         "tie_frames": ties,
     }
+    print(json.dumps(out), flush=True)
+
+
+def log(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def bench_bootstrap(args, abi, synth):
+    """64 bootstrap replicates as contiguous windows (n_r ~ U[500, 4000]) of
+    a 512-node, 4000-frame C-alpha-like trajectory, all 130,816 pairs each:
+    one autotuned, hipGraph-captured persistent launch vs one launch per
+    replicate (the reference runs its pair loop once per replicate)."""
+    N, nT, d, k, R = 512, 4000, 3, args.k, 64
+    X, _ = synth.calpha_trajectory(N=N, n=nT, seed=7)
+    ab = synth.all_pairs(N)
+    P = len(ab)
+    win = synth.bootstrap_windows(R=R, n_total=nT, lo=500, hi=4000, seed=3)
+    dX = abi.DeviceArray.from_numpy(X)
+    dab = abi.DeviceArray.from_numpy(ab)
+    out_b = abi.DeviceArray((R, P), np.float32)
+    out_s = abi.DeviceArray((R, P), np.float32)
+    work = float(sum(int(l) * (int(l) - 1) for _, l in win)) * P * (8 * d + 2)
+    timer = abi.Timer()
+
+    log(f"bootstrap: {R} windows x {P} pairs, tuning")
+    t0 = time.perf_counter()
+    plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, out_b, autotune=True)
+    tune_s = time.perf_counter() - t0
+    cfg = plan.config()
+    log(f"plan {cfg}")
+    plan.launch()
+    abi.sync()
+    log("batched warmup done")
+    batched = []
+    for _ in range(args.steps):
+        timer.start()
+        plan.launch()
+        timer.stop()
+        batched.append(timer.ms())
+        log(f"batched replay {batched[-1]:.1f} ms")
+    ties = plan.tie_frames()
+
+    per_rep = []
+    for _ in range(max(1, args.steps)):
+        timer.start()
+        for r, (off, ln) in enumerate(win):
+            abi.ksg_mi(dX, X.shape[1], nT, d, int(ln), dab, P, k, out_s,
+                       frame_offset=int(off), out_offset=r * P)
+        timer.stop()
+        per_rep.append(timer.ms())
+        log(f"per-replicate launches {per_rep[-1]:.1f} ms")
+    same = bool(np.array_equal(out_b.numpy(), out_s.numpy()))
+    tb, ts = float(np.min(batched)), float(np.min(per_rep))
+    out = {
+        "metric": "bootstrap replicate node-pairs/s (batched graph launch)",
+        "value": R * P / (tb * 1e-3), "unit": "replicate node-pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": tb, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"bootstrap GC: {N} nodes, d={d}, k={k}, {R} windows "
+                               f"n_r ~ U[500,4000] of a {nT}-frame trajectory, {P} pairs each",
+                   "plan": cfg, "tune_s": tune_s},
+        "per_replicate_ms": ts, "batched_ms": tb, "speedup_vs_per_replicate": ts / tb,
+        "outputs_identical": same, "tie_frames": ties,
+        "roofline": {"bound": "valu", "achieved": work / (tb * 1e-3) / 1e12,
+                     "peak": VALU_PEAK / 1e12, "unit": "T lane-ops/s",
+                     "frac": work / (tb * 1e-3) / VALU_PEAK},
+    }
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_apsp(args, abi, synth):
+    """Hedetniemi APSP with paths on the contact graph of the gc workload:
+    A_ij = -log(GC_ij) where the mean C-alpha distance is below 7.5 A,
+    maxPathLength = hop diameter + 1."""
+    N, n, d, k = args.nodes, args.frames, 3, args.k
+    X, mean = synth.calpha_trajectory(N=N, n=n, seed=1)
+    pairs = synth.contact_pairs(mean)
+    dX = abi.DeviceArray.from_numpy(X)
+    dab = abi.DeviceArray.from_numpy(pairs)
+    dmi = abi.DeviceArray((len(pairs),), np.float32)
+    abi.ksg_mi(dX, X.shape[1], n, d, n, dab, len(pairs), k, dmi)
+    gc = abi.gc_from_mi(dmi.numpy(), d)
+    A = synth.contact_adjacency(gc, pairs, N)
+    L = synth.hop_diameter(A) + 1
+    log(f"apsp: N={N}, {len(pairs)} contact pairs, L={L}")
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    dP = abi.DeviceArray((N * L * N,), np.int32)
+    timer = abi.Timer()
+    abi.hedetniemi(dA, N, L, 1e-4, dH, dHi, dP)
+    abi.sync()
+    timer.start()
+    abi.hedetniemi_lengths(dA, N, L, dH)
+    timer.stop()
+    t_len = timer.ms()
+    log(f"lengths {t_len:.1f} ms")
+    timer.start()
+    abi.hedetniemi(dA, N, L, 1e-4, dH, dHi, dP)
+    timer.stop()
+    t_all = timer.ms()
+    log(f"with paths {t_all:.1f} ms")
+    ops = 2.0 * N ** 3 * L
+    out = {"metric": "Hedetniemi APSP with paths", "value": t_all, "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"N={N} contact graph (-log GC, mean distance < 7.5 A), "
+                                  f"L={L}", "contact_pairs": int(len(pairs))},
+           "lengths_ms": t_len, "backtrack_ms": t_all - t_len,
+           "roofline": {"bound": "valu", "achieved": ops / (t_len * 1e-3) / 1e12,
+                        "peak": VALU_PEAK / 1e12, "unit": "T lane-ops/s",
+                        "frac": ops / (t_len * 1e-3) / VALU_PEAK,
+                        "work": "2 N^3 lane-ops (add, min) per iteration"}}
     print(json.dumps(out), flush=True)
 
 

@@ -91,6 +91,35 @@ int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride,
  * on this thread (resolved on the device in libstdc++ std::sort order). */
 int nsx_ksg_last_tie_frames(long long *count);
 
+/* Launch shape for the eager KSG calls above: queries per lane (1 or 2),
+ * LDS frame tile (64, 128 or 256), waves per workgroup (1..8), workgroups per
+ * CU (0 = as many as fit). qpl = 0 clears the override (tuned or default
+ * shapes apply again). Results do not depend on the shape. No reference
+ * counterpart (its launch shape is fixed, mutual_information_gpu.cu:1055-1062). */
+int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu);
+
+/* ---- batched plans: one hipGraph launch per replay ----------------------
+ * A plan fixes X, the windows, the pairs and the output of
+ * nsx_ksg_mi_windows, owns its psi table, window table and tie scratch, and
+ * captures [tie-counter reset, persistent KSG launch] into a hipGraph, so a
+ * replay is one hipGraphLaunch. autotune != 0 times candidate launch shapes
+ * (queries per lane x LDS frame tile x waves per workgroup) on a sample of
+ * the items first and keeps the fastest (cached per (d, k, window-length
+ * bucket); persisted to the file named by NSX_TUNE_CACHE when set).
+ * No reference counterpart: the reference runs one pair loop per bootstrap
+ * replicate (docs/tutorial3_bootstrapping.rst:125). */
+typedef struct nsx_ksg_plan_s *nsx_ksg_plan;
+int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld,
+                        long long dstride, int d, const int *h_win_off,
+                        const int *h_win_n, int R, const int *d_ab, long long P,
+                        int k, float *d_mi_out, int autotune);
+int nsx_ksg_plan_launch(nsx_ksg_plan plan, void *stream);
+/* Tie frames of the last replay (synchronises the device). */
+int nsx_ksg_plan_tie_frames(nsx_ksg_plan plan, long long *count);
+int nsx_ksg_plan_config(nsx_ksg_plan plan, int *qpl, int *chunk, int *wpb,
+                        long long *blocks, double *tune_ms);
+int nsx_ksg_plan_destroy(nsx_ksg_plan plan);
+
 /* GC transform of generalized_correlation_{cpu,gpu}.cpp:8-33 on host data:
  * h_gc[p] = MI <= 0 ? 0 : (float)sqrt(1 - (float)exp(-(2/(float)d)*MI)),
  * evaluated in double like the reference. */

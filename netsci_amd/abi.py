@@ -37,6 +37,13 @@ SIGNATURES = {
     "nsx_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP, _VP],
     "nsx_ksg_mi_windows": [_VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I, _VP, _VP],
     "nsx_ksg_last_tie_frames": [_LLP],
+    "nsx_ksg_set_config": [_I, _I, _I, _I],
+    "nsx_ksg_plan_create": [ctypes.POINTER(_VP), _VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I,
+                            _VP, _I],
+    "nsx_ksg_plan_launch": [_VP, _VP],
+    "nsx_ksg_plan_tie_frames": [_VP, _LLP],
+    "nsx_ksg_plan_config": [_VP, _IP, _IP, _IP, _LLP, ctypes.POINTER(ctypes.c_double)],
+    "nsx_ksg_plan_destroy": [_VP],
     "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
     "nsx_psi_host": [_VP, _I],
     "nsx_cpu_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _I],
@@ -129,8 +136,13 @@ class DeviceArray:
             pass
 
 
-def ksg_mi(dX, ld, dstride, d, n, dab, P, k, dmi, dcounts=None, stream=None):
-    check(lib().nsx_ksg_mi(dX.ptr, ld, dstride, d, n, dab.ptr, P, k, dmi.ptr,
+def ksg_mi(dX, ld, dstride, d, n, dab, P, k, dmi, dcounts=None, stream=None, frame_offset=0,
+           out_offset=0):
+    """nsx_ksg_mi; frame_offset shifts X to a window, out_offset (floats)
+    shifts the MI output."""
+    x = ctypes.c_void_p(dX.ptr.value + 4 * frame_offset)
+    o = ctypes.c_void_p(dmi.ptr.value + 4 * out_offset)
+    check(lib().nsx_ksg_mi(x, ld, dstride, d, n, dab.ptr, P, k, o,
                            dcounts.ptr if dcounts is not None else None, stream), "nsx_ksg_mi")
 
 
@@ -141,6 +153,53 @@ def ksg_mi_windows(dX, ld, dstride, d, windows, dab, P, k, dmi, stream=None):
     check(lib().nsx_ksg_mi_windows(dX.ptr, ld, dstride, d,
                                    off.ctypes.data_as(_IP), ln.ctypes.data_as(_IP), len(w),
                                    dab.ptr, P, k, dmi.ptr, stream), "nsx_ksg_mi_windows")
+
+
+def set_config(qpl=0, chunk=128, wpb=4, blocks_per_cu=0):
+    """Force the eager KSG launch shape (qpl=0 restores tuned/default shapes)."""
+    check(lib().nsx_ksg_set_config(qpl, chunk, wpb, blocks_per_cu), "set config")
+
+
+class KsgPlan:
+    """A hipGraph-captured batched KSG launch (nsx_ksg_plan_*)."""
+
+    def __init__(self, dX, ld, dstride, d, windows, dab, P, k, dmi, autotune=True):
+        w = np.ascontiguousarray(windows, np.int32).reshape(-1, 2)
+        self._off = np.ascontiguousarray(w[:, 0])
+        self._len = np.ascontiguousarray(w[:, 1])
+        self.ptr = ctypes.c_void_p()
+        check(lib().nsx_ksg_plan_create(ctypes.byref(self.ptr), dX.ptr, ld, dstride, d,
+                                        self._off.ctypes.data_as(_IP),
+                                        self._len.ctypes.data_as(_IP), len(w), dab.ptr, P, k,
+                                        dmi.ptr, int(bool(autotune))), "plan create")
+
+    def launch(self, stream=None):
+        check(lib().nsx_ksg_plan_launch(self.ptr, stream), "plan launch")
+
+    def tie_frames(self):
+        c = ctypes.c_longlong(0)
+        check(lib().nsx_ksg_plan_tie_frames(self.ptr, ctypes.byref(c)), "plan ties")
+        return c.value
+
+    def config(self):
+        q, c, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        b, t = ctypes.c_longlong(), ctypes.c_double()
+        check(lib().nsx_ksg_plan_config(self.ptr, ctypes.byref(q), ctypes.byref(c),
+                                        ctypes.byref(w), ctypes.byref(b), ctypes.byref(t)),
+              "plan config")
+        return {"queries_per_lane": q.value, "frame_tile": c.value, "waves_per_workgroup": w.value,
+                "workgroups": b.value, "tune_ms": t.value}
+
+    def close(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().nsx_ksg_plan_destroy(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def last_tie_frames():

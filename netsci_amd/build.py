@@ -19,7 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["nsx_runtime.hip", "ksg.hip", "apsp.hip"]
+HIP_SOURCES = ["nsx_runtime.hip", "ksg.hip", "ksg_d1.hip", "ksg_d2.hip", "ksg_d3.hip",
+               "apsp.hip"]
 HOST_SOURCES = ["host_cpu.cpp"]
 PY_SOURCES = ["host/netcalc.cpp", "host/py_module.cpp"]
 

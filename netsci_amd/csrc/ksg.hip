@@ -1,467 +1,301 @@
-// KSG (Kraskov algorithm 2) mutual information over node pairs on gfx950.
+// Host side of the KSG engine: argument checks, the (window, pair) item
+// order, launch configuration, the eager entry points, hipGraph-captured
+// plans for repeated batched launches, and the launch-shape autotuner.
 //
-// Reference semantics: netcalc/src/mutual_information_cpu.cpp:6-121 (the
-// CPU path, which fixes tie order through std::sort) and the GPU driver
-// netcalc/src/mutual_information_gpu.cu:1007-1079. Output equals the
-// reference CPU path bit-for-bit: the same float32 distances and `<=`
-// counts, the k-th/(k+1)-th tie resolved in libstdc++ order, and the psi
-// epilogue summed sequentially in frame order.
-//
-// Mapping (one persistent launch for every (window, pair) item):
-//   * a wavefront owns one item at a time (items are dealt round-robin to
-//     the resident waves, longest windows first) and walks its query frames
-//     in blocks of 64 (lane = query frame);
-//   * the item's frames stream through a wave-private LDS chunk (SoA per
-//     coordinate, NaN-padded) and are read back as 16-byte broadcasts, so
-//     every lane compares its query against the same frame j;
-//   * pass 1 keeps, per lane, the k+1 smallest joint max-norm distances to
-//     frames j != i in registers (sorted; insertion is one v_med3_f32 plus
-//     index selects per slot, taken only when some lane's distance beats its
-//     current (k+1)-th);
-//   * a lane whose k-th and (k+1)-th distances are equal rebuilds the
-//     frame's distance row in scratch and selects the k nearest in
-//     libstdc++ std::sort order (sort_select.h);
-//   * eps_x / eps_y come from the k selected frames; pass 2 counts
-//     dx <= eps_x and dy <= eps_y over all frames (self included, then minus
-//     one, as the reference does);
-//   * the wave adds psi[n_x] + psi[n_y] over its 64 lanes in lane order,
-//     block after block, so the float32 sum runs in frame order.
-// No MFMA: the work is compare/max/select, not a contraction. The roofline
-// is FP32 VALU.
+// Reference: the pair loops netcalc/src/mutual_information.cpp:103-172 and
+// generalized_correlation.cpp:104-173 call mutualInformationGpu
+// (mutual_information_gpu.cu:1007-1079) once per pair — a map rebuild, psi
+// table, four allocations, four blocking copies and one launch each. Here all
+// P pairs (times R windows) are one persistent launch of ksg_kernel.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "../../include/netsci_hip.h"
+#include "ksg_kernel.h"
 #include "nsx_internal.h"
-#include "sort_select.h"
 
 namespace nsx {
 namespace {
 
-constexpr int kWave = 64;
-constexpr int kChunk = 128;         // frames per LDS chunk
-constexpr int kWavesPerBlock = 4;   // independent waves per workgroup
-constexpr int kMaxK = 16;
+using ksg::Args;
+using ksg::KernelFn;
+using ksg::kWave;
 
-struct KsgArgs {
-    const float *X;
-    long long ld;        // row stride (floats)
-    long long dstride;   // stride between coordinate dimensions (floats)
-    const int *ab;       // P x 2 node indices
-    long long P;
-    const int *win_off;  // per processing slot: first frame
-    const int *win_n;    // per processing slot: frame count
-    const int *win_rep;  // per processing slot: output replicate index
-    int R;
-    int k;
-    const float *psi;    // psi[0..max n]
-    float inv_k;         // (float)(1.0 / (float)k)
-    float *mi_out;       // [rep][P]
-    int *counts_out;     // nullable: [p][frame][2]
-    KeyIdx *scratch;     // per wave: scratch_stride records
-    long long scratch_stride;
-    unsigned long long *stats;  // [0] tie frames
+// Launch shape. The autotuner chooses among these fields.
+struct Config {
+    int qpl = 1;     // queries per lane (1 or 2): 64*qpl query frames per sweep
+    int chunk = 128; // frames per LDS tile (64, 128, 256)
+    int wpb = 4;     // waves per workgroup (2, 4, 8)
+    int occ = 0;     // workgroups per CU (0: as many as fit)
 };
 
-// Make a wave-uniform 64-bit value provably uniform (SGPR) for the compiler.
-__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long v) {
-    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
-}
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void wave_sync_global() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// Marginal max-norm distance |q - x|_inf over D coordinates.
-template <int D>
-__device__ __forceinline__ float marg(const float (&q)[D], const float (&x)[D]) {
-    float r = fabsf(q[0] - x[0]);
-#pragma unroll
-    for (int d = 1; d < D; ++d) r = fmaxf(r, fabsf(q[d] - x[d]));
-    return r;
-}
-
-// Marginal distance from query q to frame j read from global memory.
-template <int D>
-__device__ __forceinline__ float marg_global(const float (&q)[D], const float *x,
-                                             long long ds, int j) {
-    float v[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) v[d] = x[d * ds + j];
-    return marg<D>(q, v);
-}
-
-// Insert (c, j) into the ascending heap h[0..K] (caller checked c < h[K]).
-template <int K>
-__device__ __forceinline__ void heap_insert(float (&h)[K + 1], int (&id)[K + 1],
-                                            float c, int j) {
-#pragma unroll
-    for (int m = K; m >= 1; --m) {
-        const bool lp = c < h[m - 1];
-        const bool lc = c < h[m];
-        id[m] = lp ? id[m - 1] : (lc ? j : id[m]);
-        h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+KernelFn kernel_for(int d, int k, int qpl) {
+    switch (d) {
+        case 1: return ksg::kernel_d1(k, qpl);
+        case 2: return ksg::kernel_d2(k, qpl);
+        case 3: return ksg::kernel_d3(k, qpl);
+        default: return nullptr;
     }
-    const bool l0 = c < h[0];
-    id[0] = l0 ? j : id[0];
-    h[0] = fminf(h[0], c);
 }
 
-template <int D>
-struct Chunk {
-    float *s;  // [2D][kChunk]: a-coordinates then b-coordinates
-    __device__ __forceinline__ float4 get(int dim, int jj) const {
-        return *reinterpret_cast<const float4 *>(s + dim * kChunk + jj);
-    }
+// Validated problem description with the processing order of the windows.
+struct Problem {
+    const float *X = nullptr;
+    long long ld = 0, dstride = 0;
+    int d = 0, k = 0, R = 0, nmax = 0;
+    const int *ab = nullptr;
+    long long P = 0;
+    double work = 0;            // sum over items of n(n-1), for reporting
+    std::vector<int> host_win;  // [off(R) | n(R) | rep(R)] in processing order
 };
 
-// Stage frames [c0, c0 + kChunk) of both rows into the wave's LDS chunk.
-// Frames at or beyond n are NaN so that every comparison with them fails.
-template <int D>
-__device__ __forceinline__ void stage_chunk(const Chunk<D> &ch, const float *xa,
-                                            const float *xb, long long ds, int c0,
-                                            int n, int lane) {
-    const float qnan = __builtin_nanf("");
-#pragma unroll
-    for (int h = 0; h < kChunk / kWave; ++h) {
-        const int f = c0 + h * kWave + lane;
-        const bool ok = f < n;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            const float va = ok ? xa[d * ds + f] : qnan;
-            const float vb = ok ? xb[d * ds + f] : qnan;
-            ch.s[d * kChunk + h * kWave + lane] = va;
-            ch.s[(D + d) * kChunk + h * kWave + lane] = vb;
-        }
-    }
-    wave_sync();
-}
-
-// Pass 1 over one staged chunk: keep the k+1 nearest frames j != i.
-template <int D, int K, bool kSelf>
-__device__ __forceinline__ void pass1_chunk(const Chunk<D> &ch, int cnt4, int c0, int i,
-                                            const float (&qa)[D], const float (&qb)[D],
-                                            float (&h)[K + 1], int (&id)[K + 1]) {
-    for (int jj = 0; jj < cnt4; jj += 4) {
-        float4 A[D], B[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            A[d] = ch.get(d, jj);
-            B[d] = ch.get(D + d, jj);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            float xa[D], xb[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                xa[d] = u == 0 ? A[d].x : u == 1 ? A[d].y : u == 2 ? A[d].z : A[d].w;
-                xb[d] = u == 0 ? B[d].x : u == 1 ? B[d].y : u == 2 ? B[d].z : B[d].w;
-            }
-            float dz = fmaxf(marg<D>(qa, xa), marg<D>(qb, xb));
-            const int j = c0 + jj + u;
-            if (kSelf) dz = (j == i) ? __builtin_nanf("") : dz;
-            if (dz < h[K]) heap_insert<K>(h, id, dz, j);
-        }
-    }
-}
-
-// Pass 2 over one staged chunk: marginal counts within eps (self included).
-template <int D>
-__device__ __forceinline__ void pass2_chunk(const Chunk<D> &ch, int cnt4,
-                                            const float (&qa)[D], const float (&qb)[D],
-                                            float ex, float ey, int &cx, int &cy) {
-    for (int jj = 0; jj < cnt4; jj += 4) {
-        float4 A[D], B[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            A[d] = ch.get(d, jj);
-            B[d] = ch.get(D + d, jj);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            float xa[D], xb[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                xa[d] = u == 0 ? A[d].x : u == 1 ? A[d].y : u == 2 ? A[d].z : A[d].w;
-                xb[d] = u == 0 ? B[d].x : u == 1 ? B[d].y : u == 2 ? B[d].z : B[d].w;
-            }
-            cx += marg<D>(qa, xa) <= ex ? 1 : 0;
-            cy += marg<D>(qb, xb) <= ey ? 1 : 0;
-        }
-    }
-}
-
-// Resolve the eps of every lane flagged `tie`: rebuild the frame's joint
-// distance row into scratch and select the k+1 first entries in libstdc++
-// std::sort order, then take eps over positions 1..k
-// (mutual_information_cpu.cpp:48-77).
-template <int D>
-__device__ unsigned resolve_ties(bool tie, const float (&qa)[D], const float (&qb)[D],
-                                 const float *xa, const float *xb, long long ds, int n, int k,
-                                 KeyIdx *scratch, int lane, float &ex, float &ey) {
-    unsigned long long tmask = __ballot(tie);
-    const unsigned count = (unsigned)__popcll(tmask);
-    while (tmask) {
-        const int t = __builtin_amdgcn_readfirstlane(__ffsll((long long)tmask) - 1);
-        float tqa[D], tqb[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-            tqa[d] = __shfl(qa[d], t);
-            tqb[d] = __shfl(qb[d], t);
-        }
-        for (int j = lane; j < n; j += kWave) {
-            float va[D], vb[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                va[d] = xa[d * ds + j];
-                vb[d] = xb[d * ds + j];
-            }
-            KeyIdx r;
-            r.key = fmaxf(marg<D>(tqa, va), marg<D>(tqb, vb));
-            r.idx = j;
-            scratch[j] = r;
-        }
-        wave_sync_global();
-        float rea = 0.0f, reb = 0.0f;
-        if (lane == 0) {
-            sort_prefix(scratch, n, k + 1);
-            for (int m = 1; m <= k; ++m) {
-                const int j = scratch[m].idx;
-                rea = fmaxf(rea, marg_global<D>(tqa, xa, ds, j));
-                reb = fmaxf(reb, marg_global<D>(tqb, xb, ds, j));
-            }
-        }
-        rea = __shfl(rea, 0);
-        reb = __shfl(reb, 0);
-        ex = (lane == t) ? rea : ex;
-        ey = (lane == t) ? reb : ey;
-        wave_sync_global();
-        tmask &= tmask - 1;
-    }
-    return count;
-}
-
-// Every wave walks items w = wave_gid, wave_gid + nwaves, ... All loop
-// bounds and item fields are wave-uniform scalars; the only lane-divergent
-// regions are the heap insertion, the tie rebuild's single-lane selection
-// and the per-lane loads/stores.
-template <int D, int K>
-__global__ __launch_bounds__(kWave *kWavesPerBlock) void ksg_kernel(KsgArgs a) {
-    __shared__ __attribute__((aligned(16))) float lds[kWavesPerBlock][2 * D * kChunk];
-    __shared__ float sterm[kWavesPerBlock][kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    Chunk<D> ch{lds[wib]};
-    float *terms = sterm[wib];
-    const long long nwaves = (long long)gridDim.x * kWavesPerBlock;
-    const long long wave_gid = (long long)blockIdx.x * kWavesPerBlock + wib;
-    KeyIdx *scratch = a.scratch + wave_gid * a.scratch_stride;
-    const long long total = (long long)a.R * a.P;
-    const long long ds = a.dstride;
-    unsigned long long ties = 0;
-
-    for (long long w = wave_gid; w < total; w += nwaves) {
-        const int slot = (int)(w / a.P);
-        const long long p = w - (long long)slot * a.P;
-        const int off = a.win_off[slot];
-        const int n = a.win_n[slot];
-        const float *xa = a.X + (long long)a.ab[2 * p] * a.ld + off;
-        const float *xb = a.X + (long long)a.ab[2 * p + 1] * a.ld + off;
-
-        float acc = 0.0f;
-        for (int q0 = 0; q0 < n; q0 += kWave) {
-            const int i = q0 + lane;
-            const bool valid = i < n;
-            const int iq = valid ? i : 0;
-            float qa[D], qb[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                qa[d] = xa[d * ds + iq];
-                qb[d] = xb[d * ds + iq];
-            }
-            float h[K + 1];
-            int id[K + 1];
-#pragma unroll
-            for (int m = 0; m <= K; ++m) {
-                h[m] = __builtin_inff();
-                id[m] = 0;
-            }
-            // Pass 1: k+1 nearest by joint distance.
-            for (int c0 = 0; c0 < n; c0 += kChunk) {
-                stage_chunk<D>(ch, xa, xb, ds, c0, n, lane);
-                const int cnt4 = (min(kChunk, n - c0) + 3) & ~3;
-                const bool self = (c0 < q0 + kWave) && (q0 < c0 + kChunk);
-                if (self) pass1_chunk<D, K, true>(ch, cnt4, c0, i, qa, qb, h, id);
-                else pass1_chunk<D, K, false>(ch, cnt4, c0, i, qa, qb, h, id);
-                wave_sync();
-            }
-            // eps over the k nearest; boundary ties take the exact path.
-            const bool tie = valid && (h[K - 1] == h[K]);
-            float ex = 0.0f, ey = 0.0f;
-#pragma unroll
-            for (int m = 0; m < K; ++m) {
-                ex = fmaxf(ex, marg_global<D>(qa, xa, ds, id[m]));
-                ey = fmaxf(ey, marg_global<D>(qb, xb, ds, id[m]));
-            }
-            ties += resolve_ties<D>(tie, qa, qb, xa, xb, ds, n, K, scratch, lane, ex, ey);
-            // Pass 2: marginal counts.
-            int cx = 0, cy = 0;
-            for (int c0 = 0; c0 < n; c0 += kChunk) {
-                stage_chunk<D>(ch, xa, xb, ds, c0, n, lane);
-                const int cnt4 = (min(kChunk, n - c0) + 3) & ~3;
-                pass2_chunk<D>(ch, cnt4, qa, qb, ex, ey, cx, cy);
-                wave_sync();
-            }
-            const int nx = cx - 1, ny = cy - 1;
-            terms[lane] = valid ? a.psi[nx] + a.psi[ny] : 0.0f;
-            if (a.counts_out && valid) {
-                a.counts_out[(p * n + i) * 2 + 0] = nx;
-                a.counts_out[(p * n + i) * 2 + 1] = ny;
-            }
-            wave_sync();
-            // Sequential float32 sum in frame order (every lane runs the same
-            // sum over LDS broadcasts).
-            const int lim = min(kWave, n - q0);
-            for (int l = 0; l < lim; ++l) acc += terms[l];
-            wave_sync();
-        }
-        const float avg = acc / (float)n;
-        const float mi = a.psi[n] + a.psi[a.k] - a.inv_k - avg;
-        if (lane == 0) a.mi_out[(long long)a.win_rep[slot] * a.P + p] = mi;
-    }
-    if (lane == 0 && ties) atomicAdd(a.stats, ties);
-}
-
-using KernelFn = void (*)(KsgArgs);
-
-template <int D, int K>
-struct KTable {
-    static void fill(KernelFn *t) {
-        t[K] = ksg_kernel<D, K>;
-        KTable<D, K - 1>::fill(t);
-    }
-};
-template <int D>
-struct KTable<D, 0> {
-    static void fill(KernelFn *) {}
-};
-
-KernelFn kernel_for(int d, int k) {
-    static KernelFn tab[4][kMaxK + 1] = {};
-    static bool init = false;
-    if (!init) {
-        KTable<1, kMaxK>::fill(tab[1]);
-        KTable<2, kMaxK>::fill(tab[2]);
-        KTable<3, kMaxK>::fill(tab[3]);
-        init = true;
-    }
-    if (d < 1 || d > 3 || k < 1 || k > kMaxK) return nullptr;
-    return tab[d][k];
-}
-
-thread_local long long g_last_ties = 0;
-
-int run_ksg(const float *X, long long ld, long long dstride, int d,
-            const std::vector<int> &off, const std::vector<int> &nfr,
-            const int *ab, long long P, int k, float *mi_out, int *counts_out,
-            hipStream_t stream) {
-    const int R = (int)off.size();
-    NSX_REQUIRE(X && ab && mi_out, "ksg: null pointer argument");
+int make_problem(Problem &pb, const float *X, long long ld, long long dstride, int d,
+                 const int *off, const int *nfr, int R, const int *ab, long long P, int k) {
+    NSX_REQUIRE(X && ab, "ksg: null pointer argument");
     NSX_REQUIRE(d >= 1 && d <= 3, "ksg: the GPU path supports d = 1, 2 or 3");
-    NSX_REQUIRE(k >= 1 && k <= kMaxK, "ksg: the GPU path supports 1 <= k <= 16");
+    NSX_REQUIRE(k >= 1 && k <= ksg::kMaxK, "ksg: the GPU path supports 1 <= k <= 16");
     NSX_REQUIRE(P >= 0 && R >= 1, "ksg: bad pair or window count");
     NSX_REQUIRE(ld > 0 && dstride > 0, "ksg: bad strides");
+    NSX_REQUIRE((long long)(d - 1) * dstride + dstride <= ld,
+                "ksg: d * dstride exceeds the row length");
     int nmax = 0;
+    double work = 0;
     for (int r = 0; r < R; ++r) {
         NSX_REQUIRE(nfr[r] > k, "ksg: every window needs more than k frames");
         NSX_REQUIRE(off[r] >= 0 && (long long)off[r] + nfr[r] <= dstride,
                     "ksg: window exceeds the frames of a row");
         nmax = std::max(nmax, nfr[r]);
+        work += (double)nfr[r] * (nfr[r] - 1) * (double)P;
     }
-    NSX_REQUIRE((long long)(d - 1) * dstride + dstride <= ld,
-                "ksg: d * dstride exceeds the row length");
+    // Longest windows first, so the biggest items start first.
+    std::vector<int> order(R);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nfr[x] > nfr[y]; });
+    pb.host_win.assign(3 * R, 0);
+    for (int s = 0; s < R; ++s) {
+        pb.host_win[s] = off[order[s]];
+        pb.host_win[R + s] = nfr[order[s]];
+        pb.host_win[2 * R + s] = order[s];
+    }
+    pb.X = X;
+    pb.ld = ld;
+    pb.dstride = dstride;
+    pb.d = d;
+    pb.k = k;
+    pb.R = R;
+    pb.nmax = nmax;
+    pb.ab = ab;
+    pb.P = P;
+    pb.work = work;
+    return kOk;
+}
+
+// Resident waves a launch may use (8 per SIMD, 4 SIMDs per CU): sizes the
+// tie scratch of plans and the tuner.
+long long max_waves() { return (long long)cu_count() * 32; }
+
+struct Shape {
+    KernelFn fn = nullptr;
+    long long blocks = 0;
+    size_t lds = 0;
+};
+
+int shape_for(const Problem &pb, const Config &c, long long items, Shape &s) {
+    s.fn = kernel_for(pb.d, pb.k, c.qpl);
+    NSX_REQUIRE(s.fn, "ksg: no kernel for (d, k, queries per lane)");
+    NSX_REQUIRE((c.chunk == 64 || c.chunk == 128 || c.chunk == 256) && c.wpb >= 1 &&
+                    c.wpb <= ksg::kMaxWavesPerBlock,
+                "ksg: bad launch configuration");
+    s.lds = ksg::lds_bytes(pb.d, c.qpl, c.chunk, c.wpb);
+    int per_cu = 0;
+    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s.fn,
+                                                         kWave * c.wpb, s.lds),
+            "occupancy query");
+    per_cu = std::max(per_cu, 1);
+    if (c.occ > 0) per_cu = std::min(per_cu, c.occ);
+    long long blocks = (long long)cu_count() * per_cu;
+    blocks = std::min<long long>(blocks, (items + c.wpb - 1) / c.wpb);
+    blocks = std::min<long long>(blocks, max_waves() / c.wpb);
+    s.blocks = std::max<long long>(blocks, 1);
+    return kOk;
+}
+
+// Device buffers one launch needs besides its inputs and outputs.
+struct Buffers {
+    float *psi = nullptr;
+    int *win = nullptr;
+    KeyIdx *scratch = nullptr;
+    unsigned long long *stats = nullptr;
+};
+
+Args make_args(const Problem &pb, const Config &c, const Buffers &b, float *mi_out,
+               int *counts_out, long long P_used) {
+    Args a;
+    a.X = pb.X;
+    a.ld = pb.ld;
+    a.dstride = pb.dstride;
+    a.ab = pb.ab;
+    a.P = P_used;
+    a.win_off = b.win;
+    a.win_n = b.win + pb.R;
+    a.win_rep = b.win + 2 * pb.R;
+    a.R = pb.R;
+    a.k = pb.k;
+    a.psi = b.psi;
+    a.inv_k = (float)(1.0 / (float)pb.k);
+    a.mi_out = mi_out;
+    a.counts_out = counts_out;
+    a.scratch = b.scratch;
+    a.scratch_stride = pb.nmax;
+    a.stats = b.stats;
+    a.chunk = c.chunk;
+    a.wpb = c.wpb;
+    return a;
+}
+
+int launch(const Shape &s, const Config &c, const Args &a, hipStream_t stream) {
+    hipLaunchKernelGGL(s.fn, dim3((unsigned)s.blocks), dim3(kWave * c.wpb), s.lds, stream, a);
+    NSX_HIP(hipGetLastError(), "ksg launch");
+    return kOk;
+}
+
+// ---- tuning cache ---------------------------------------------------------
+
+std::mutex g_tune_mutex;
+std::map<std::string, Config> g_tuned;
+bool g_tune_loaded = false;
+Config g_override;  // chunk == 0: no override
+bool g_has_override = false;
+
+std::string tune_key(const Problem &pb) {
+    // Bucket the mean window length by powers of two; batched and single
+    // windows tune separately.
+    long long nsum = 0;
+    for (int s = 0; s < pb.R; ++s) nsum += pb.host_win[pb.R + s];
+    const long long nmean = nsum / pb.R;
+    int bucket = 0;
+    while ((1LL << (bucket + 1)) <= nmean) ++bucket;
+    std::ostringstream os;
+    os << "d" << pb.d << "_k" << pb.k << "_n2^" << bucket << (pb.R > 1 ? "_batched" : "_plain");
+    return os.str();
+}
+
+void load_tune_cache() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *path = std::getenv("NSX_TUNE_CACHE");
+    if (!path) return;
+    std::ifstream f(path);
+    std::string key;
+    Config c;
+    while (f >> key >> c.qpl >> c.chunk >> c.wpb >> c.occ) g_tuned[key] = c;
+}
+
+void store_tune(const std::string &key, const Config &c) {
+    g_tuned[key] = c;
+    const char *path = std::getenv("NSX_TUNE_CACHE");
+    if (!path) return;
+    std::ofstream f(path, std::ios::app);
+    f << key << " " << c.qpl << " " << c.chunk << " " << c.wpb << " " << c.occ << "\n";
+}
+
+Config config_for(const Problem &pb) {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (g_has_override) return g_override;
+    load_tune_cache();
+    auto it = g_tuned.find(tune_key(pb));
+    if (it != g_tuned.end()) return it->second;
+    return Config{};
+}
+
+// Time each candidate on a sample of the problem's items (every window, the
+// first Ps pairs — enough items to fill every resident wave twice) and keep
+// the fastest. Candidates: queries per lane {1, 2} x LDS tile {64, 128, 256}
+// frames x waves per workgroup {4, 8}.
+int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &best,
+             double *best_ms) {
+    const long long target_items = 2 * max_waves();
+    long long Ps = std::min<long long>(pb.P, std::max<long long>(1, target_items / pb.R));
+    float *d_out = (float *)workspace(kWsScreen1, (size_t)pb.R * Ps * sizeof(float));
+    if (!d_out) return kHipError;
+    hipEvent_t e0, e1;
+    NSX_HIP(hipEventCreate(&e0), "tune event");
+    NSX_HIP(hipEventCreate(&e1), "tune event");
+    double best_t = 1e300;
+    for (int qpl : {1, 2})
+        for (int chunk : {64, 128, 256})
+            for (int wpb : {4, 8}) {
+                Config c;
+                c.qpl = qpl;
+                c.chunk = chunk;
+                c.wpb = wpb;
+                Shape s;
+                int rc = shape_for(pb, c, (long long)pb.R * Ps, s);
+                if (rc) return rc;
+                const Args a = make_args(pb, c, b, d_out, nullptr, Ps);
+                rc = launch(s, c, a, stream);  // warm-up (code object load, caches)
+                if (rc) return rc;
+                NSX_HIP(hipEventRecord(e0, stream), "tune record");
+                rc = launch(s, c, a, stream);
+                if (rc) return rc;
+                NSX_HIP(hipEventRecord(e1, stream), "tune record");
+                NSX_HIP(hipEventSynchronize(e1), "tune sync");
+                float ms = 0;
+                NSX_HIP(hipEventElapsedTime(&ms, e0, e1), "tune elapsed");
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = c;
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (best_ms) *best_ms = best_t;
+    return kOk;
+}
+
+// ---- eager path -------------------------------------------------------------
+
+thread_local long long g_last_ties = 0;
+
+int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t stream) {
+    NSX_REQUIRE(mi_out, "ksg: null output pointer");
     int rc = ensure_device();
     if (rc) return rc;
     g_last_ties = 0;
-    if (P == 0) return kOk;
-
-    // Processing order: longest windows first (the biggest items start first).
-    std::vector<int> order(R);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return nfr[x] > nfr[y]; });
-    std::vector<int> host_win(3 * R);
-    for (int s = 0; s < R; ++s) {
-        host_win[s] = off[order[s]];
-        host_win[R + s] = nfr[order[s]];
-        host_win[2 * R + s] = order[s];
-    }
-
-    KernelFn fn = kernel_for(d, k);
-    NSX_REQUIRE(fn, "ksg: no kernel for (d, k)");
-    int per_cu = 0;
-    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn,
-                                                         kWave * kWavesPerBlock, 0),
-            "occupancy query");
-    per_cu = std::max(per_cu, 1);
-    const unsigned long long items = (unsigned long long)R * (unsigned long long)P;
-    long long blocks = (long long)cu_count() * per_cu;
-    blocks = std::min<long long>(blocks, (long long)((items + kWavesPerBlock - 1) / kWavesPerBlock));
-    blocks = std::max<long long>(blocks, 1);
-    const long long waves = blocks * kWavesPerBlock;
-
-    std::vector<float> psi(nmax + 1);
-    psi_table(psi.data(), nmax);
-
-    float *d_psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));
-    unsigned long long *d_ctr = (unsigned long long *)workspace(kWsCounter, 64);
-    KeyIdx *d_scr = (KeyIdx *)workspace(kWsTieScratch, (size_t)waves * nmax * sizeof(KeyIdx));
-    int *d_win = (int *)workspace(kWsWindows, host_win.size() * sizeof(int));
-    if (!d_psi || !d_ctr || !d_scr || !d_win) return kHipError;
-    NSX_HIP(hipMemcpyAsync(d_psi, psi.data(), psi.size() * sizeof(float),
-                           hipMemcpyHostToDevice, stream), "psi upload");
-    NSX_HIP(hipMemcpyAsync(d_win, host_win.data(), host_win.size() * sizeof(int),
-                           hipMemcpyHostToDevice, stream), "window upload");
-    rc = fill_u32(d_ctr, 0u, 16, stream);  // [0..1]: tie-frame count
+    if (pb.P == 0) return kOk;
+    const Config c = config_for(pb);
+    Shape s;
+    rc = shape_for(pb, c, (long long)pb.R * pb.P, s);
     if (rc) return rc;
-
-    KsgArgs args;
-    args.X = X;
-    args.ld = ld;
-    args.dstride = dstride;
-    args.ab = ab;
-    args.P = P;
-    args.win_off = d_win;
-    args.win_n = d_win + R;
-    args.win_rep = d_win + 2 * R;
-    args.R = R;
-    args.k = k;
-    args.psi = d_psi;
-    args.inv_k = (float)(1.0 / (float)k);
-    args.mi_out = mi_out;
-    args.counts_out = counts_out;
-    args.scratch = d_scr;
-    args.scratch_stride = nmax;
-    args.stats = d_ctr;
-    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(kWave * kWavesPerBlock), 0,
-                       stream, args);
-    NSX_HIP(hipGetLastError(), "ksg launch");
-    // Tie statistics for the run record (synchronises the stream).
+    std::vector<float> psi(pb.nmax + 1);
+    psi_table(psi.data(), pb.nmax);
+    Buffers b;
+    b.psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));
+    b.stats = (unsigned long long *)workspace(kWsCounter, 64);
+    b.scratch = (KeyIdx *)workspace(kWsTieScratch,
+                                    (size_t)s.blocks * c.wpb * pb.nmax * sizeof(KeyIdx));
+    b.win = (int *)workspace(kWsWindows, pb.host_win.size() * sizeof(int));
+    if (!b.psi || !b.stats || !b.scratch || !b.win) return kHipError;
+    NSX_HIP(hipMemcpyAsync(b.psi, psi.data(), psi.size() * sizeof(float),
+                           hipMemcpyHostToDevice, stream), "psi upload");
+    NSX_HIP(hipMemcpyAsync(b.win, pb.host_win.data(), pb.host_win.size() * sizeof(int),
+                           hipMemcpyHostToDevice, stream), "window upload");
+    rc = fill_u32(b.stats, 0u, 2, stream);
+    if (rc) return rc;
+    rc = launch(s, c, make_args(pb, c, b, mi_out, counts_out, pb.P), stream);
+    if (rc) return rc;
     unsigned long long ties = 0;
-    NSX_HIP(hipMemcpyAsync(&ties, d_ctr, sizeof(ties), hipMemcpyDeviceToHost, stream),
+    NSX_HIP(hipMemcpyAsync(&ties, b.stats, sizeof(ties), hipMemcpyDeviceToHost, stream),
             "tie count readback");
     NSX_HIP(hipStreamSynchronize(stream), "ksg sync");
     g_last_ties = (long long)ties;
@@ -471,7 +305,35 @@ int run_ksg(const float *X, long long ld, long long dstride, int d,
 }  // namespace
 }  // namespace nsx
 
+// ---- plans: hipGraph-captured batched launches --------------------------------
+
+struct nsx_ksg_plan_s {
+    nsx::Problem pb;
+    nsx::Config cfg;
+    nsx::Shape shape;
+    nsx::Buffers buf;
+    float *mi_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double tune_ms = 0;
+};
+
 using namespace nsx;
+
+namespace {
+
+void destroy_plan(nsx_ksg_plan_s *p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->buf.psi);
+    (void)hipFree(p->buf.win);
+    (void)hipFree(p->buf.scratch);
+    (void)hipFree(p->buf.stats);
+    delete p;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -479,9 +341,11 @@ int nsx_ksg_mi(const float *d_X, long long ld, long long dstride, int d, int n,
                const int *d_ab, long long P, int k, float *d_mi_out, int *d_counts_out,
                void *stream) {
     clear_error();
-    std::vector<int> off{0}, nfr{n};
-    return run_ksg(d_X, ld, dstride, d, off, nfr, d_ab, P, k, d_mi_out, d_counts_out,
-                   (hipStream_t)stream);
+    const int off = 0;
+    Problem pb;
+    int rc = make_problem(pb, d_X, ld, dstride, d, &off, &n, 1, d_ab, P, k);
+    if (rc) return rc;
+    return run_eager(pb, d_mi_out, d_counts_out, (hipStream_t)stream);
 }
 
 int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
@@ -489,13 +353,160 @@ int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
                        long long P, int k, float *d_mi_out, void *stream) {
     clear_error();
     NSX_REQUIRE(h_win_off && h_win_n && R >= 1, "ksg windows: bad window arrays");
-    std::vector<int> off(h_win_off, h_win_off + R), nfr(h_win_n, h_win_n + R);
-    return run_ksg(d_X, ld, dstride, d, off, nfr, d_ab, P, k, d_mi_out, nullptr,
-                   (hipStream_t)stream);
+    Problem pb;
+    int rc = make_problem(pb, d_X, ld, dstride, d, h_win_off, h_win_n, R, d_ab, P, k);
+    if (rc) return rc;
+    return run_eager(pb, d_mi_out, nullptr, (hipStream_t)stream);
 }
 
 int nsx_ksg_last_tie_frames(long long *count) {
     if (count) *count = g_last_ties;
+    return kOk;
+}
+
+int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu) {
+    clear_error();
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (qpl == 0) {
+        g_has_override = false;
+        return kOk;
+    }
+    NSX_REQUIRE((qpl == 1 || qpl == 2) && (chunk == 64 || chunk == 128 || chunk == 256) &&
+                    wpb >= 1 && wpb <= ksg::kMaxWavesPerBlock && blocks_per_cu >= 0,
+                "nsx_ksg_set_config: bad configuration");
+    g_override = Config{qpl, chunk, wpb, blocks_per_cu};
+    g_has_override = true;
+    return kOk;
+}
+
+int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long long dstride,
+                        int d, const int *h_win_off, const int *h_win_n, int R,
+                        const int *d_ab, long long P, int k, float *d_mi_out, int autotune) {
+    clear_error();
+    NSX_REQUIRE(plan && d_mi_out && h_win_off && h_win_n && R >= 1 && P >= 1,
+                "nsx_ksg_plan_create: invalid arguments");
+    *plan = nullptr;
+    int rc = ensure_device();
+    if (rc) return rc;
+    auto *p = new nsx_ksg_plan_s();
+    rc = make_problem(p->pb, d_X, ld, dstride, d, h_win_off, h_win_n, R, d_ab, P, k);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    const Problem &pb = p->pb;
+    p->mi_out = d_mi_out;
+    // Plan-owned buffers (allocated once, reused by every replay).
+    std::vector<float> psi(pb.nmax + 1);
+    psi_table(psi.data(), pb.nmax);
+    const size_t scratch_bytes = (size_t)max_waves() * pb.nmax * sizeof(KeyIdx);
+    hipError_t e = hipMalloc(&p->buf.psi, psi.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&p->buf.win, pb.host_win.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->buf.scratch, scratch_bytes);
+    if (e == hipSuccess) e = hipMalloc(&p->buf.stats, 64);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->buf.psi, psi.data(), psi.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->buf.win, pb.host_win.data(), pb.host_win.size() * sizeof(int),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("nsx_ksg_plan_create: ") + hipGetErrorString(e));
+        destroy_plan(p);
+        return kHipError;
+    }
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_error(std::string("capture stream: ") + hipGetErrorString(e));
+        destroy_plan(p);
+        return kHipError;
+    }
+    if (autotune) {
+        Config best;
+        rc = nsx::autotune(pb, p->buf, cs, best, &p->tune_ms);
+        if (rc) {
+            (void)hipStreamDestroy(cs);
+            destroy_plan(p);
+            return rc;
+        }
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        store_tune(tune_key(pb), best);
+        p->cfg = best;
+    } else {
+        p->cfg = config_for(pb);
+    }
+    rc = shape_for(pb, p->cfg, (long long)pb.R * pb.P, p->shape);
+    if (!rc && (size_t)p->shape.blocks * p->cfg.wpb * pb.nmax * sizeof(KeyIdx) > scratch_bytes) {
+        set_error("nsx_ksg_plan_create: tie scratch too small for the chosen shape");
+        rc = kInternal;
+    }
+    // Capture: reset the tie counter (a fill kernel, not a memset node), then
+    // the persistent launch.
+    if (!rc) {
+        e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            rc = fill_u32(p->buf.stats, 0u, 2, cs);
+            if (!rc) rc = launch(p->shape, p->cfg,
+                                 make_args(pb, p->cfg, p->buf, d_mi_out, nullptr, pb.P), cs);
+            hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
+            if (!rc && e2 != hipSuccess) {
+                set_error(std::string("end capture: ") + hipGetErrorString(e2));
+                rc = kHipError;
+            }
+        } else {
+            set_error(std::string("begin capture: ") + hipGetErrorString(e));
+            rc = kHipError;
+        }
+    }
+    if (!rc) {
+        e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            set_error(std::string("graph instantiate: ") + hipGetErrorString(e));
+            rc = kHipError;
+        }
+    }
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamDestroy(cs);
+    if (rc) {
+        destroy_plan(p);
+        return rc;
+    }
+    *plan = p;
+    return kOk;
+}
+
+int nsx_ksg_plan_launch(nsx_ksg_plan plan, void *stream) {
+    clear_error();
+    NSX_REQUIRE(plan && plan->exec, "nsx_ksg_plan_launch: no plan");
+    NSX_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream), "hipGraphLaunch");
+    return kOk;
+}
+
+int nsx_ksg_plan_tie_frames(nsx_ksg_plan plan, long long *count) {
+    clear_error();
+    NSX_REQUIRE(plan && count, "nsx_ksg_plan_tie_frames: invalid arguments");
+    unsigned long long v = 0;
+    NSX_HIP(hipDeviceSynchronize(), "plan sync");
+    NSX_HIP(hipMemcpy(&v, plan->buf.stats, sizeof(v), hipMemcpyDeviceToHost), "tie readback");
+    *count = (long long)v;
+    return kOk;
+}
+
+int nsx_ksg_plan_config(nsx_ksg_plan plan, int *qpl, int *chunk, int *wpb, long long *blocks,
+                        double *tune_ms) {
+    clear_error();
+    NSX_REQUIRE(plan, "nsx_ksg_plan_config: no plan");
+    if (qpl) *qpl = plan->cfg.qpl;
+    if (chunk) *chunk = plan->cfg.chunk;
+    if (wpb) *wpb = plan->cfg.wpb;
+    if (blocks) *blocks = plan->shape.blocks;
+    if (tune_ms) *tune_ms = plan->tune_ms;
+    return kOk;
+}
+
+int nsx_ksg_plan_destroy(nsx_ksg_plan plan) {
+    if (plan) (void)hipStreamSynchronize(nullptr);
+    destroy_plan(plan);
     return kOk;
 }
 

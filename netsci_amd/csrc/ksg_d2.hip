@@ -1,0 +1,11 @@
+// KSG kernel instances for d = 2 (one translation unit per dimension so
+// that the instances compile in parallel).
+#include "ksg_kernel.h"
+
+namespace nsx {
+namespace ksg {
+
+KernelFn kernel_d2(int k, int qpl) { return kernel_lookup<2>(k, qpl); }
+
+}  // namespace ksg
+}  // namespace nsx

@@ -1,0 +1,409 @@
+// KSG (Kraskov algorithm 2) mutual-information kernel for gfx950.
+//
+// Reference semantics: netcalc/src/mutual_information_cpu.cpp:6-121 (the CPU
+// path, which fixes tie order through std::sort); the GPU driver it replaces
+// is netcalc/src/mutual_information_gpu.cu:1007-1079. Output equals the
+// reference CPU path bit-for-bit: the same float32 distances and `<=`
+// counts, the k-th/(k+1)-th tie resolved in libstdc++ order, and the psi
+// epilogue summed sequentially in frame order.
+//
+// Mapping (one persistent launch for every (window, pair) item):
+//   * a wavefront owns one item at a time (items are dealt round-robin to
+//     the resident waves, longest windows first) and walks its query frames
+//     in blocks of 64*QPL (lane l owns queries q0 + l and, for QPL = 2,
+//     q0 + 64 + l);
+//   * the item's frames stream through a wave-private LDS tile of `chunk`
+//     frames (SoA per coordinate, NaN-padded) and are read back as 16-byte
+//     broadcasts, so every lane compares its queries against the same j;
+//   * pass 1 keeps, per query, the k+1 smallest joint max-norm distances to
+//     frames j != i in registers (sorted; insertion is one v_med3_f32 plus
+//     index selects per slot, taken only when some lane's distance beats its
+//     current (k+1)-th);
+//   * a query whose k-th and (k+1)-th distances are equal rebuilds its
+//     distance row in scratch and selects the k nearest in libstdc++
+//     std::sort order (sort_select.h);
+//   * eps_x / eps_y come from the k selected frames; pass 2 counts
+//     dx <= eps_x and dy <= eps_y over all frames (self included, then minus
+//     one, as the reference does);
+//   * the wave adds psi[n_x] + psi[n_y] over its queries in frame order.
+// No MFMA: the work is compare/max/select, not a contraction. The roofline
+// is FP32 VALU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sort_select.h"
+
+namespace nsx {
+namespace ksg {
+
+constexpr int kWave = 64;
+constexpr int kMaxK = 16;
+constexpr int kMaxChunk = 256;
+constexpr int kMaxWavesPerBlock = 8;
+
+struct Args {
+    const float *X;
+    long long ld;        // row stride (floats)
+    long long dstride;   // stride between coordinate dimensions (floats)
+    const int *ab;       // P x 2 node indices
+    long long P;
+    const int *win_off;  // per processing slot: first frame
+    const int *win_n;    // per processing slot: frame count
+    const int *win_rep;  // per processing slot: output replicate index
+    int R;
+    int k;
+    const float *psi;    // psi[0..max n]
+    float inv_k;         // (float)(1.0 / (float)k)
+    float *mi_out;       // [rep][P]
+    int *counts_out;     // nullable: [p][frame][2] (single window only)
+    KeyIdx *scratch;     // per wave: scratch_stride records
+    long long scratch_stride;
+    unsigned long long *stats;  // [0] tie frames
+    int chunk;           // frames per LDS tile: 64, 128 or 256
+    int wpb;             // waves per workgroup
+};
+
+using KernelFn = void (*)(Args);
+
+// Dynamic LDS bytes a launch needs.
+inline size_t lds_bytes(int d, int qpl, int chunk, int wpb) {
+    return (size_t)wpb * (size_t)(2 * d * chunk + kWave * qpl) * sizeof(float);
+}
+
+// Kernel instances, one table per coordinate dimension (ksg_d{1,2,3}.hip).
+KernelFn kernel_d1(int k, int qpl);
+KernelFn kernel_d2(int k, int qpl);
+KernelFn kernel_d3(int k, int qpl);
+
+#if defined(__HIPCC__)
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void wave_sync_global() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Marginal max-norm distance |q - x|_inf over D coordinates.
+template <int D>
+__device__ __forceinline__ float marg(const float (&q)[D], const float (&x)[D]) {
+    float r = fabsf(q[0] - x[0]);
+#pragma unroll
+    for (int d = 1; d < D; ++d) r = fmaxf(r, fabsf(q[d] - x[d]));
+    return r;
+}
+
+// Marginal distance from query q to frame j read from global memory.
+template <int D>
+__device__ __forceinline__ float marg_global(const float (&q)[D], const float *x,
+                                             long long ds, int j) {
+    float v[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d] = x[d * ds + j];
+    return marg<D>(q, v);
+}
+
+// Insert (c, j) into the ascending heap h[0..K] (caller checked c < h[K]).
+template <int K>
+__device__ __forceinline__ void heap_insert(float (&h)[K + 1], int (&id)[K + 1], float c,
+                                            int j) {
+#pragma unroll
+    for (int m = K; m >= 1; --m) {
+        const bool lp = c < h[m - 1];
+        const bool lc = c < h[m];
+        id[m] = lp ? id[m - 1] : (lc ? j : id[m]);
+        h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+    }
+    const bool l0 = c < h[0];
+    id[0] = l0 ? j : id[0];
+    h[0] = l0 ? c : h[0];
+}
+
+template <int D>
+struct Tile {
+    float *s;   // [2D][chunk]: a-coordinates then b-coordinates
+    int chunk;
+    __device__ __forceinline__ float4 get(int dim, int jj) const {
+        return *reinterpret_cast<const float4 *>(s + dim * chunk + jj);
+    }
+};
+
+// Stage frames [c0, c0 + chunk) of both rows into the wave's LDS tile.
+// Frames at or beyond n are NaN so that every comparison with them fails.
+template <int D>
+__device__ __forceinline__ void stage_tile(const Tile<D> &t, const float *xa, const float *xb,
+                                           long long ds, int c0, int n, int lane) {
+    const float qnan = __builtin_nanf("");
+    for (int h = 0; h < t.chunk; h += kWave) {
+        const int f = c0 + h + lane;
+        const bool ok = f < n;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const float va = ok ? xa[d * ds + f] : qnan;
+            const float vb = ok ? xb[d * ds + f] : qnan;
+            t.s[d * t.chunk + h + lane] = va;
+            t.s[(D + d) * t.chunk + h + lane] = vb;
+        }
+    }
+    wave_sync();
+}
+
+template <int D>
+__device__ __forceinline__ void unpack(const float4 (&A)[D], const float4 (&B)[D], int u,
+                                       float (&xa)[D], float (&xb)[D]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        xa[d] = u == 0 ? A[d].x : u == 1 ? A[d].y : u == 2 ? A[d].z : A[d].w;
+        xb[d] = u == 0 ? B[d].x : u == 1 ? B[d].y : u == 2 ? B[d].z : B[d].w;
+    }
+}
+
+// Pass 1 over one staged tile: keep the k+1 nearest frames j != i per query.
+template <int D, int K, int QPL, bool kSelf>
+__device__ __forceinline__ void pass1_tile(const Tile<D> &t, int cnt4, int c0,
+                                           const int (&i)[QPL], const float (&qa)[QPL][D],
+                                           const float (&qb)[QPL][D], float (&h)[QPL][K + 1],
+                                           int (&id)[QPL][K + 1]) {
+    for (int jj = 0; jj < cnt4; jj += 4) {
+        float4 A[D], B[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            A[d] = t.get(d, jj);
+            B[d] = t.get(D + d, jj);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float xa[D], xb[D];
+            unpack<D>(A, B, u, xa, xb);
+            const int j = c0 + jj + u;
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                float dz = fmaxf(marg<D>(qa[q], xa), marg<D>(qb[q], xb));
+                if (kSelf) dz = (j == i[q]) ? __builtin_nanf("") : dz;
+                if (dz < h[q][K]) heap_insert<K>(h[q], id[q], dz, j);
+            }
+        }
+    }
+}
+
+// Pass 2 over one staged tile: marginal counts within eps (self included).
+template <int D, int QPL>
+__device__ __forceinline__ void pass2_tile(const Tile<D> &t, int cnt4,
+                                           const float (&qa)[QPL][D], const float (&qb)[QPL][D],
+                                           const float (&ex)[QPL], const float (&ey)[QPL],
+                                           int (&cx)[QPL], int (&cy)[QPL]) {
+    for (int jj = 0; jj < cnt4; jj += 4) {
+        float4 A[D], B[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            A[d] = t.get(d, jj);
+            B[d] = t.get(D + d, jj);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float xa[D], xb[D];
+            unpack<D>(A, B, u, xa, xb);
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                cx[q] += marg<D>(qa[q], xa) <= ex[q] ? 1 : 0;
+                cy[q] += marg<D>(qb[q], xb) <= ey[q] ? 1 : 0;
+            }
+        }
+    }
+}
+
+// Resolve the eps of every lane flagged `tie`: rebuild the frame's joint
+// distance row into scratch and select the k+1 first entries in libstdc++
+// std::sort order, then take eps over positions 1..k
+// (mutual_information_cpu.cpp:48-77). Returns the number of tie frames.
+template <int D>
+__device__ unsigned resolve_ties(bool tie, const float (&qa)[D], const float (&qb)[D],
+                                 const float *xa, const float *xb, long long ds, int n, int k,
+                                 KeyIdx *scratch, int lane, float &ex, float &ey) {
+    unsigned long long tmask = __ballot(tie);
+    const unsigned count = (unsigned)__popcll(tmask);
+    while (tmask) {
+        const int t = __builtin_amdgcn_readfirstlane(__ffsll((long long)tmask) - 1);
+        float tqa[D], tqb[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            tqa[d] = __shfl(qa[d], t);
+            tqb[d] = __shfl(qb[d], t);
+        }
+        for (int j = lane; j < n; j += kWave) {
+            float va[D], vb[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                va[d] = xa[d * ds + j];
+                vb[d] = xb[d * ds + j];
+            }
+            KeyIdx r;
+            r.key = fmaxf(marg<D>(tqa, va), marg<D>(tqb, vb));
+            r.idx = j;
+            scratch[j] = r;
+        }
+        wave_sync_global();
+        float rea = 0.0f, reb = 0.0f;
+        if (lane == 0) {
+            sort_prefix(scratch, n, k + 1);
+            for (int m = 1; m <= k; ++m) {
+                const int j = scratch[m].idx;
+                rea = fmaxf(rea, marg_global<D>(tqa, xa, ds, j));
+                reb = fmaxf(reb, marg_global<D>(tqb, xb, ds, j));
+            }
+        }
+        rea = __shfl(rea, 0);
+        reb = __shfl(reb, 0);
+        ex = (lane == t) ? rea : ex;
+        ey = (lane == t) ? reb : ey;
+        wave_sync_global();
+        tmask &= tmask - 1;
+    }
+    return count;
+}
+
+// Every wave walks items w = wave_gid, wave_gid + nwaves, ... All loop
+// bounds and item fields are wave-uniform scalars; the only lane-divergent
+// regions are the heap insertion, the tie rebuild's single-lane selection
+// and the per-lane loads/stores.
+template <int D, int K, int QPL>
+__global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
+    extern __shared__ __attribute__((aligned(16))) float dyn_lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int chunk = a.chunk;
+    float *wave_lds = dyn_lds + wib * (2 * D * chunk + kWave * QPL);
+    const Tile<D> tile{wave_lds, chunk};
+    float *terms = wave_lds + 2 * D * chunk;
+    const long long nwaves = (long long)gridDim.x * a.wpb;
+    const long long wave_gid = (long long)blockIdx.x * a.wpb + wib;
+    KeyIdx *scratch = a.scratch + wave_gid * a.scratch_stride;
+    const long long total = (long long)a.R * a.P;
+    const long long ds = a.dstride;
+    unsigned long long ties = 0;
+
+    for (long long w = wave_gid; w < total; w += nwaves) {
+        const int slot = (int)(w / a.P);
+        const long long p = w - (long long)slot * a.P;
+        const int off = a.win_off[slot];
+        const int n = a.win_n[slot];
+        const float *xa = a.X + (long long)a.ab[2 * p] * a.ld + off;
+        const float *xb = a.X + (long long)a.ab[2 * p + 1] * a.ld + off;
+
+        float acc = 0.0f;
+        for (int q0 = 0; q0 < n; q0 += kWave * QPL) {
+            int i[QPL];
+            bool valid[QPL];
+            float qa[QPL][D], qb[QPL][D];
+            float h[QPL][K + 1];
+            int id[QPL][K + 1];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                i[q] = q0 + q * kWave + lane;
+                valid[q] = i[q] < n;
+                const int iq = valid[q] ? i[q] : 0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    qa[q][d] = xa[d * ds + iq];
+                    qb[q][d] = xb[d * ds + iq];
+                }
+#pragma unroll
+                for (int m = 0; m <= K; ++m) {
+                    h[q][m] = __builtin_inff();
+                    id[q][m] = 0;
+                }
+            }
+            // Pass 1: k+1 nearest by joint distance.
+            for (int c0 = 0; c0 < n; c0 += chunk) {
+                stage_tile<D>(tile, xa, xb, ds, c0, n, lane);
+                const int cnt4 = (min(chunk, n - c0) + 3) & ~3;
+                const bool self = (c0 < q0 + kWave * QPL) && (q0 < c0 + chunk);
+                if (self) pass1_tile<D, K, QPL, true>(tile, cnt4, c0, i, qa, qb, h, id);
+                else pass1_tile<D, K, QPL, false>(tile, cnt4, c0, i, qa, qb, h, id);
+                wave_sync();
+            }
+            // eps over the k nearest; boundary ties take the exact path.
+            float ex[QPL], ey[QPL];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                ex[q] = 0.0f;
+                ey[q] = 0.0f;
+#pragma unroll
+                for (int m = 0; m < K; ++m) {
+                    ex[q] = fmaxf(ex[q], marg_global<D>(qa[q], xa, ds, id[q][m]));
+                    ey[q] = fmaxf(ey[q], marg_global<D>(qb[q], xb, ds, id[q][m]));
+                }
+                const bool tie = valid[q] && (h[q][K - 1] == h[q][K]);
+                ties += resolve_ties<D>(tie, qa[q], qb[q], xa, xb, ds, n, K, scratch, lane,
+                                        ex[q], ey[q]);
+            }
+            // Pass 2: marginal counts.
+            int cx[QPL], cy[QPL];
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) cx[q] = cy[q] = 0;
+            for (int c0 = 0; c0 < n; c0 += chunk) {
+                stage_tile<D>(tile, xa, xb, ds, c0, n, lane);
+                const int cnt4 = (min(chunk, n - c0) + 3) & ~3;
+                pass2_tile<D, QPL>(tile, cnt4, qa, qb, ex, ey, cx, cy);
+                wave_sync();
+            }
+#pragma unroll
+            for (int q = 0; q < QPL; ++q) {
+                const int nx = cx[q] - 1, ny = cy[q] - 1;
+                terms[q * kWave + lane] = valid[q] ? a.psi[nx] + a.psi[ny] : 0.0f;
+                if (a.counts_out && valid[q]) {
+                    a.counts_out[(p * n + i[q]) * 2 + 0] = nx;
+                    a.counts_out[(p * n + i[q]) * 2 + 1] = ny;
+                }
+            }
+            wave_sync();
+            // Sequential float32 sum in frame order (every lane runs the same
+            // sum over LDS broadcasts).
+            const int lim = min(kWave * QPL, n - q0);
+            for (int l = 0; l < lim; ++l) acc += terms[l];
+            wave_sync();
+        }
+        const float avg = acc / (float)n;
+        const float mi = a.psi[n] + a.psi[a.k] - a.inv_k - avg;
+        if (lane == 0) a.mi_out[(long long)a.win_rep[slot] * a.P + p] = mi;
+    }
+    if (lane == 0 && ties) atomicAdd(a.stats, ties);
+}
+
+template <int D, int K>
+struct Instances {
+    static void fill(KernelFn (*t)[3]) {
+        t[K][1] = ksg_kernel<D, K, 1>;
+        t[K][2] = ksg_kernel<D, K, 2>;
+        Instances<D, K - 1>::fill(t);
+    }
+};
+template <int D>
+struct Instances<D, 0> {
+    static void fill(KernelFn (*)[3]) {}
+};
+
+// Table lookup shared by the per-dimension translation units.
+template <int D>
+struct InstanceTable {
+    KernelFn t[kMaxK + 1][3] = {};
+    InstanceTable() { Instances<D, kMaxK>::fill(t); }
+};
+
+template <int D>
+KernelFn kernel_lookup(int k, int qpl) {
+    static const InstanceTable<D> tab;  // thread-safe static initialisation
+    if (k < 1 || k > kMaxK || qpl < 1 || qpl > 2) return nullptr;
+    return tab.t[k][qpl];
+}
+
+#endif  // __HIPCC__
+
+}  // namespace ksg
+}  // namespace nsx
