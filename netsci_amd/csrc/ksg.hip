@@ -33,7 +33,7 @@ using ksg::kWave;
 
 // Launch shape. The autotuner chooses among these fields.
 struct Config {
-    int qpl = 1;     // queries per lane (1 or 2): 64*qpl query frames per sweep
+    int qpl = 2;     // queries per lane (1 or 2): 64*qpl query frames per sweep
     int chunk = 128; // frames per LDS tile (64, 128, 256)
     int wpb = 4;     // waves per workgroup (2, 4, 8)
     int occ = 0;     // workgroups per CU (0: as many as fit)

@@ -68,7 +68,8 @@ using KernelFn = void (*)(Args);
 
 // Dynamic LDS bytes a launch needs.
 inline size_t lds_bytes(int d, int qpl, int chunk, int wpb) {
-    return (size_t)wpb * (size_t)(2 * d * chunk + kWave * qpl) * sizeof(float);
+    (void)chunk;
+    return (size_t)wpb * (size_t)(2 * d * kMaxChunk + kWave * qpl) * sizeof(float);
 }
 
 // Kernel instances, one table per coordinate dimension (ksg_d{1,2,3}.hip).
@@ -125,12 +126,15 @@ __device__ __forceinline__ void heap_insert(float (&h)[K + 1], int (&id)[K + 1],
     h[0] = l0 ? c : h[0];
 }
 
+// A wave's LDS tile: 2D rows (a-coordinates then b-coordinates) of a fixed
+// stride kMaxChunk, so every row address is the tile base plus an immediate
+// offset; `chunk` (64, 128 or 256) frames are staged per pass.
 template <int D>
 struct Tile {
-    float *s;   // [2D][chunk]: a-coordinates then b-coordinates
+    float *s;
     int chunk;
     __device__ __forceinline__ float4 get(int dim, int jj) const {
-        return *reinterpret_cast<const float4 *>(s + dim * chunk + jj);
+        return *reinterpret_cast<const float4 *>(s + dim * kMaxChunk + jj);
     }
 };
 
@@ -147,8 +151,8 @@ __device__ __forceinline__ void stage_tile(const Tile<D> &t, const float *xa, co
         for (int d = 0; d < D; ++d) {
             const float va = ok ? xa[d * ds + f] : qnan;
             const float vb = ok ? xb[d * ds + f] : qnan;
-            t.s[d * t.chunk + h + lane] = va;
-            t.s[(D + d) * t.chunk + h + lane] = vb;
+            t.s[d * kMaxChunk + h + lane] = va;
+            t.s[(D + d) * kMaxChunk + h + lane] = vb;
         }
     }
     wave_sync();
@@ -278,9 +282,9 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int chunk = a.chunk;
-    float *wave_lds = dyn_lds + wib * (2 * D * chunk + kWave * QPL);
+    float *wave_lds = dyn_lds + wib * (2 * D * kMaxChunk + kWave * QPL);
     const Tile<D> tile{wave_lds, chunk};
-    float *terms = wave_lds + 2 * D * chunk;
+    float *terms = wave_lds + 2 * D * kMaxChunk;
     const long long nwaves = (long long)gridDim.x * a.wpb;
     const long long wave_gid = (long long)blockIdx.x * a.wpb + wib;
     KeyIdx *scratch = a.scratch + wave_gid * a.scratch_stride;
