@@ -128,6 +128,41 @@ int nsx_gc_from_mi_host(const float *h_mi, long long P, int d, float *h_gc);
 /* psi table of psi.cpp:6-19: h_psi[0..n]. */
 int nsx_psi_host(float *h_psi, int n);
 
+/* ---- learned pair screener ---------------------------------------------
+ * No reference counterpart: the reference computes KSG for every pair. These
+ * entry points let all-pairs GC run the exact KSG path only on pairs a small
+ * MLP predicts may exceed a GC threshold, plus an audit sample; the values
+ * reported for computed pairs are always exact KSG.
+ *
+ * nsx_screen_features: 16 bf16 features per pair (see screener.hip) from the
+ * trajectory's covariance (d = 3, X in the layout above). d_feat: P x 16
+ * uint16 (bf16 bits). */
+int nsx_screen_features(const float *d_X, long long ld, long long dstride, int N,
+                        int n, const int *d_ab, long long P,
+                        unsigned short *d_feat, void *stream);
+/* MLP 16 -> 64 -> 64 -> 1 (ReLU) with FP32 master parameters in one array of
+ * nsx_mlp_param_count() floats: W1[16][64], b1[64], W2[64][64], b2[64],
+ * w3[64], b3. Products run on bf16 MFMA with FP32 accumulation. */
+int nsx_mlp_param_count(int *count);
+int nsx_mlp_forward(const float *d_params, const unsigned short *d_feat,
+                    long long P, float *d_out, void *stream);
+/* Gradient of mean_i w_i (out_i - y_i)^2 over a batch of B samples (summed
+ * deterministically) and the loss. */
+int nsx_mlp_grad(const float *d_params, const unsigned short *d_feat,
+                 const float *d_y, const float *d_w, long long B, float *d_grad,
+                 float *d_loss, void *stream);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) on the FP32 master parameters. */
+int nsx_mlp_adam(float *d_params, const float *d_grad, float *d_m, float *d_v,
+                 int step, float lr, void *stream);
+/* Pairs with prediction >= threshold, plus a hashed audit sample of about
+ * audit_rate of the others, compacted into d_ab_out (pairs) and d_idx_out
+ * (their indices); d_flag[p] = 1 candidate, 2 audit, 0 skipped. Returns the
+ * selected and audit counts on the host (synchronises the stream). */
+int nsx_screen_select(const float *d_pred, long long P, float threshold,
+                      float audit_rate, unsigned seed, const int *d_ab,
+                      int *d_ab_out, long long *d_idx_out, unsigned char *d_flag,
+                      long long *h_n_selected, long long *h_n_audit, void *stream);
+
 /* ---- host (CPU platform) implementations -------------------------------
  * The CPU side of the `platform` flag (platform.h:9-10). Same results as the
  * GPU path; threads over pairs. */

@@ -46,6 +46,12 @@ SIGNATURES = {
     "nsx_ksg_plan_destroy": [_VP],
     "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
     "nsx_psi_host": [_VP, _I],
+    "nsx_screen_features": [_VP, _LL, _LL, _I, _I, _VP, _LL, _VP, _VP],
+    "nsx_mlp_param_count": [_IP],
+    "nsx_mlp_forward": [_VP, _VP, _LL, _VP, _VP],
+    "nsx_mlp_grad": [_VP, _VP, _VP, _VP, _LL, _VP, _VP, _VP],
+    "nsx_mlp_adam": [_VP, _VP, _VP, _VP, _I, _F, _VP],
+    "nsx_screen_select": [_VP, _LL, _F, _F, ctypes.c_uint, _VP, _VP, _VP, _VP, _LLP, _LLP, _VP],
     "nsx_cpu_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _I],
     "nsx_select_prefix_host": [_VP, _I, _I, _VP, _IP],
     "nsx_hedetniemi": [_VP, _I, _I, _F, _VP, _VP, _VP, _VP],
@@ -231,6 +237,40 @@ def cpu_ksg_mi(X, ab, k, n, d, threads=0, dstride=None):
     check(lib().nsx_cpu_ksg_mi(_ptr(X), X.shape[1], dstride or n, d, n, _ptr(ab), len(ab), k,
                                _ptr(out), threads), "cpu ksg")
     return out
+
+
+def screen_features(dX, ld, dstride, N, n, dab, P, dfeat, stream=None):
+    check(lib().nsx_screen_features(dX.ptr, ld, dstride, N, n, dab.ptr, P, dfeat.ptr, stream),
+          "nsx_screen_features")
+
+
+def mlp_param_count():
+    c = ctypes.c_int(0)
+    lib().nsx_mlp_param_count(ctypes.byref(c))
+    return c.value
+
+
+def mlp_forward(dparams, dfeat, P, dout, stream=None):
+    check(lib().nsx_mlp_forward(dparams.ptr, dfeat.ptr, P, dout.ptr, stream), "nsx_mlp_forward")
+
+
+def mlp_grad(dparams, dfeat, dy, dw, B, dgrad, dloss, stream=None):
+    check(lib().nsx_mlp_grad(dparams.ptr, dfeat.ptr, dy.ptr, dw.ptr, B, dgrad.ptr, dloss.ptr,
+                             stream), "nsx_mlp_grad")
+
+
+def mlp_adam(dparams, dgrad, dm, dv, step, lr, stream=None):
+    check(lib().nsx_mlp_adam(dparams.ptr, dgrad.ptr, dm.ptr, dv.ptr, step, lr, stream),
+          "nsx_mlp_adam")
+
+
+def screen_select(dpred, P, threshold, audit_rate, seed, dab, dab_out, didx_out, dflag,
+                  stream=None):
+    ns, na = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    check(lib().nsx_screen_select(dpred.ptr, P, threshold, audit_rate, seed, dab.ptr,
+                                  dab_out.ptr, didx_out.ptr, dflag.ptr, ctypes.byref(ns),
+                                  ctypes.byref(na), stream), "nsx_screen_select")
+    return ns.value, na.value
 
 
 def hedetniemi(dA, N, L, tol, dH, dHi=None, dpaths=None, stream=None):

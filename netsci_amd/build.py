@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["nsx_runtime.hip", "ksg.hip", "ksg_d1.hip", "ksg_d2.hip", "ksg_d3.hip",
-               "apsp.hip"]
+               "apsp.hip", "screener.hip"]
 HOST_SOURCES = ["host_cpu.cpp"]
 PY_SOURCES = ["host/netcalc.cpp", "host/py_module.cpp"]
 
@@ -75,7 +75,8 @@ def build(verbose=False):
                 print(o)
     objs = [o for _, o in jobs]
     tmp = lib_path() + ".tmp"
-    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-pthread"])
+    _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs +
+         ["-pthread", "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, lib_path())
 
     import pybind11
