@@ -14,7 +14,9 @@
 //      11 log1p(|a-b|) / log1p(N)  12 ||T||_F / sqrt 3  13 max |coordinate
 //      correlation|  14 ln(n) / 10  15 1
 //    stored as bf16 (round to nearest even).
-// 2. MLP 16 -> 64 -> 64 -> 1 (ReLU), hand-written with
+// 2. MLP 16 -> 64 -> 64 -> 1 (ReLU) plus a skip of feature 5 to the
+//    output (out = net(x) + x5, so the network starts from, and corrects,
+//    the Gaussian-MI GC estimate), hand-written with
 //    v_mfma_f32_16x16x16_bf16: a wave takes 16 pairs at a time; bf16 operands,
 //    FP32 accumulation; FP32 master weights converted to bf16 per launch.
 //    Operand layouts of the 16x16x16 instruction (lane l, element e):
@@ -48,6 +50,7 @@ constexpr int kF = 16, kH = 64;
 constexpr int oW1 = 0, oB1 = 1024, oW2 = 1088, oB2 = 5184, oW3 = 5248, oB3 = 5312;
 constexpr int kParams = 5313;
 constexpr int kNodeInfo = 16;
+constexpr int kSkipFeature = 5;  // Gaussian-MI GC, added to the network output
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -285,6 +288,8 @@ __device__ void forward16(const unsigned short *__restrict__ feat, long long P, 
         for (int e = 0; e < 4; ++e) z[e] += W.b2[t];
         f.z2[t] = z;
     }
+    // Output head plus the skip connection from feature 5 (the Gaussian-MI
+    // GC): the network learns a correction to that estimate.
     for (int e = 0; e < 4; ++e) {
         float s = 0.f;
         for (int t = 0; t < 4; ++t) s += fmaxf(f.z2[t][e], 0.f) * W.w3[t];
@@ -292,7 +297,7 @@ __device__ void forward16(const unsigned short *__restrict__ feat, long long P, 
         s += __shfl_xor(s, 2);
         s += __shfl_xor(s, 4);
         s += __shfl_xor(s, 8);
-        f.out[e] = s + W.b3;
+        f.out[e] = s + W.b3 + bf2f(xs[(4 * g + e) * kF + kSkipFeature]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

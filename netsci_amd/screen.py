@@ -110,15 +110,16 @@ class Screener:
         df = abi.DeviceArray.from_numpy(feat)
         return self.predict_device(df, len(feat)).numpy()
 
-    def calibrate(self, pred, y, recall=0.999):
-        """Largest threshold on the prediction that keeps `recall` of the
-        pairs with y > GC_THRESHOLD (on held-out data)."""
+    def calibrate(self, pred, y, recall=0.999, margin=0.01):
+        """Threshold on the prediction that keeps `recall` of the held-out
+        pairs with y > GC_THRESHOLD, lowered by `margin` (GC units) against
+        the sampling noise of that quantile."""
         pos = np.sort(pred[y > GC_THRESHOLD])
         if len(pos) == 0:
             self.threshold = float(np.min(pred))
             return self.threshold
         i = int(np.floor((1.0 - recall) * len(pos)))
-        self.threshold = float(pos[max(i, 0)]) - 1e-4
+        self.threshold = float(pos[max(i, 0)]) - margin
         return self.threshold
 
     def save(self, path):
@@ -147,8 +148,19 @@ def init_params(seed=0):
     o = F * H + H
     p[o:o + H * H] = rng.standard_normal(H * H) * np.sqrt(2.0 / H)
     o += H * H + H
-    p[o:o + H] = rng.standard_normal(H) * np.sqrt(1.0 / H)
+    p[o:o + H] = rng.standard_normal(H) * 0.01  # start close to the skip (feature 5)
     return p
+
+
+def min_fraction_at_recall(score, y, recall=0.995):
+    """Smallest fraction of pairs a score threshold must keep so that
+    `recall` of the pairs with y > GC_THRESHOLD are kept (measured, not
+    calibrated: compares screening rules on the same data)."""
+    pos = np.sort(score[y > GC_THRESHOLD])
+    if len(pos) == 0:
+        return 0.0
+    t = pos[int(np.floor((1.0 - recall) * len(pos)))]
+    return float(np.mean(score >= t))
 
 
 def training_set(trajectories=8, N=1000, n=5000, pairs_per=60000, k=6, seed=100, log=None):

@@ -17,6 +17,7 @@ import numpy as np
 F, H = 16, 64
 O_W1, O_B1, O_W2, O_B2, O_W3, O_B3 = 0, 1024, 1088, 5184, 5248, 5312
 N_PARAMS = 5313
+SKIP = 5  # feature added to the output (the Gaussian-MI GC)
 
 
 def bf16_round(x):
@@ -50,15 +51,16 @@ def _q(x, bf16):
 
 
 def mlp_forward(params, x, bf16=True, keep=False):
-    """out = relu(relu(x W1 + b1) W2 + b2) . w3 + b3 with bf16 operands for
-    both matrix products (the kernels' MFMA inputs) and float accumulation."""
+    """out = relu(relu(x W1 + b1) W2 + b2) . w3 + b3 + x[5] with bf16
+    operands for both matrix products (the kernels' MFMA inputs) and float
+    accumulation."""
     W1, b1, W2, b2, w3, b3 = unpack(params)
     xq = _q(x, bf16)
     z1 = xq @ _q(W1, bf16) + b1
     h1 = np.maximum(z1, 0)
     z2 = _q(h1, bf16) @ _q(W2, bf16) + b2
     h2 = np.maximum(z2, 0)
-    out = h2 @ w3 + b3
+    out = h2 @ w3 + b3 + xq[:, SKIP]
     if keep:
         return out, (xq, z1, h1, z2, h2)
     return out
