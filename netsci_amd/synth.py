@@ -22,8 +22,6 @@ so dimension di of frame f is X[a, di * n + f].
   A_ij = -log(GC_ij) where the mean C-alpha distance is < 7.5 A, +inf
   elsewhere, 0 on the diagonal; plus the hop diameter of that graph.
 """
-from collections import deque
-
 import numpy as np
 
 
@@ -108,19 +106,12 @@ def contact_adjacency(gc_pairs, pairs, N):
 
 
 def hop_diameter(A):
-    """Largest finite BFS hop count over all sources (edges: finite A_ij)."""
-    N = A.shape[0]
-    adj = [np.nonzero(np.isfinite(A[i]) & (np.arange(N) != i))[0] for i in range(N)]
-    best = 0
-    for s in range(N):
-        dist = np.full(N, -1)
-        dist[s] = 0
-        q = deque([s])
-        while q:
-            u = q.popleft()
-            for v in adj[u]:
-                if dist[v] < 0:
-                    dist[v] = dist[u] + 1
-                    q.append(v)
-        best = max(best, int(dist.max()))
-    return best
+    """Largest finite hop count between any two nodes (edges: finite A_ij,
+    i != j), by breadth-first search from every node."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    adj = np.isfinite(A)
+    np.fill_diagonal(adj, False)
+    hops = shortest_path(csr_matrix(adj.astype(np.int8)), method="D", directed=True,
+                         unweighted=True)
+    return int(hops[np.isfinite(hops)].max())
