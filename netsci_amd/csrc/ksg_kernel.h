@@ -196,6 +196,72 @@ __device__ __forceinline__ void pass1_tile(const Tile<D> &t, int cnt4, int c0,
     }
 }
 
+// Pass 1 with deferred insertion. For each block of 32 frames every lane
+// first records, in a bit mask, the frames whose joint distance beats its
+// (k+1)-th nearest at the start of the block (one compare and one
+// add-with-carry per frame); then the wave inserts the flagged frames in
+// rounds, each lane taking its next flagged frame, re-reading that frame
+// from LDS and inserting it if it still beats the heap. A lane's flagged
+// frames are a superset of those the per-frame scan would insert, and the
+// heap's content does not depend on insertion order except for which of
+// several frames tied at the (k+1)-th distance stays — a case the tie check
+// after pass 1 hands to the exact path anyway.
+template <int D, int K, int QPL, bool kSelf>
+__device__ __forceinline__ void pass1_tile_deferred(const Tile<D> &t, int cnt, int c0,
+                                                    const int (&i)[QPL],
+                                                    const float (&qa)[QPL][D],
+                                                    const float (&qb)[QPL][D],
+                                                    float (&h)[QPL][K + 1],
+                                                    int (&id)[QPL][K + 1]) {
+    for (int jb = 0; jb < cnt; jb += 32) {
+        unsigned mask[QPL];
+        float thr[QPL];
+#pragma unroll
+        for (int q = 0; q < QPL; ++q) {
+            mask[q] = 0u;
+            thr[q] = h[q][K];
+        }
+#pragma unroll
+        for (int j4 = 0; j4 < 32; j4 += 4) {
+            float4 A[D], B[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                A[d] = t.get(d, jb + j4);
+                B[d] = t.get(D + d, jb + j4);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float xa[D], xb[D];
+                unpack<D>(A, B, u, xa, xb);
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) {
+                    float dz = fmaxf(marg<D>(qa[q], xa), marg<D>(qb[q], xb));
+                    if (kSelf) dz = (c0 + jb + j4 + u == i[q]) ? __builtin_nanf("") : dz;
+                    // bit 31 - (j4 + u) marks frame jb + j4 + u
+                    mask[q] = mask[q] + mask[q] + (dz < thr[q] ? 1u : 0u);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QPL; ++q) {
+            while (__ballot(mask[q] != 0u)) {
+                if (mask[q] != 0u) {
+                    const int u = __clz(mask[q]);  // lowest pending frame first
+                    mask[q] &= ~(0x80000000u >> u);
+                    float xa[D], xb[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        xa[d] = t.s[d * kMaxChunk + jb + u];
+                        xb[d] = t.s[(D + d) * kMaxChunk + jb + u];
+                    }
+                    const float dz = fmaxf(marg<D>(qa[q], xa), marg<D>(qb[q], xb));
+                    if (dz < h[q][K]) heap_insert<K>(h[q], id[q], dz, c0 + jb + u);
+                }
+            }
+        }
+    }
+}
+
 // Pass 2 over one staged tile: marginal counts within eps (self included).
 template <int D, int QPL>
 __device__ __forceinline__ void pass2_tile(const Tile<D> &t, int cnt4,
@@ -328,8 +394,9 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
                 stage_tile<D>(tile, xa, xb, ds, c0, n, lane);
                 const int cnt4 = (min(chunk, n - c0) + 3) & ~3;
                 const bool self = (c0 < q0 + kWave * QPL) && (q0 < c0 + chunk);
-                if (self) pass1_tile<D, K, QPL, true>(tile, cnt4, c0, i, qa, qb, h, id);
-                else pass1_tile<D, K, QPL, false>(tile, cnt4, c0, i, qa, qb, h, id);
+                const int cnt = min(chunk, n - c0);
+                if (self) pass1_tile_deferred<D, K, QPL, true>(tile, cnt, c0, i, qa, qb, h, id);
+                else pass1_tile_deferred<D, K, QPL, false>(tile, cnt, c0, i, qa, qb, h, id);
                 wave_sync();
             }
             // eps over the k nearest; boundary ties take the exact path.
