@@ -49,6 +49,8 @@ def parse():
                          "frame windows of a 512-node, 4000-frame trajectory, one graph-captured "
                          "batched launch vs one launch per replicate; apsp: Hedetniemi on the "
                          "-log(GC) contact graph of the gc workload")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only time the VALU throughput probe (v_sub, v_max3, v_fma, KSG mix)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
@@ -85,6 +87,14 @@ def main():
         abi.check(abi.lib().nsx_memcpy_d2d(dst.ptr, src.ptr, nbytes, None), "copy")
         abi.sync()
         print(json.dumps({"copy_bytes": nbytes}), flush=True)
+        return
+    if args.valu_probe:
+        res = {}
+        for kind, name in enumerate(["v_sub_f32", "v_max3_f32+v_sub_f32", "v_fma_f32", "ksg_mix"]):
+            ms, lanes = abi.probe_valu(kind)
+            res[name] = {"ms": ms, "lanes": lanes}
+            log(f"probe {name}: {ms:.2f} ms")
+        print(json.dumps({"valu_probe": res, "iters": 20000, "ops_per_iter": 32}), flush=True)
         return
     if args.workload == "bootstrap":
         return bench_bootstrap(args, abi, synth)
