@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--nodes", type=int, default=1000)
     ap.add_argument("--frames", type=int, default=2000)
     ap.add_argument("--k", type=int, default=6)
-    ap.add_argument("--cpu-pairs", type=int, default=24,
-                    help="pairs timed for the CPU baseline (0 to skip)")
+    ap.add_argument("--cpu-pairs", type=int, default=128,
+                    help="pairs timed for the CPU baseline, about 0.16 s each on one core "
+                         "(0 to skip)")
     ap.add_argument("--screen-nodes", type=int, default=4000)
     ap.add_argument("--workload", default="gc", choices=["gc", "bootstrap", "apsp", "screened"],
                     help="gc: the headline all-pairs GC pass (default); bootstrap: 64 uneven "
@@ -55,6 +56,17 @@ def parse():
                     help="only run one device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE) and exit")
     return ap.parse_args()
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
 
 
 def traffic_from_profile(name):
@@ -148,15 +160,23 @@ def main():
         import oracle
         rng = np.random.default_rng(0)
         sample = ab[rng.choice(P, size=min(args.cpu_pairs, P), replace=False)]
-        t1 = time.perf_counter()
-        ref = oracle.pairs(X, sample, k, n, d, gc=True)
-        cpu_s = time.perf_counter() - t1
+        ref = np.empty(len(sample), np.float32)
+        per_pair = np.empty(len(sample))
+        for i in range(len(sample)):
+            t1 = time.perf_counter()
+            ref[i] = oracle.pairs(X, sample[i:i + 1], k, n, d, gc=True)[0]
+            per_pair[i] = time.perf_counter() - t1
+        cpu_s = float(per_pair.sum())
         idx = {tuple(r): i for i, r in enumerate(ab)}
         match = all(ref[i] == gc[idx[tuple(r)]] for i, r in enumerate(sample))
         cpu = {"value": len(sample) / cpu_s, "unit": "GC node-pairs/s", "cores": 1,
                "kind": "port",
-               "sample": f"{len(sample)} seeded random pairs of the same workload, "
-                         f"{cpu_s:.1f} s single-thread; GPU values bit-identical: {match}"}
+               "sample": f"{len(sample)} seeded random pairs of the same workload, {cpu_s:.1f} s "
+                         f"on one core of {host_cpu_model()} ({os.cpu_count()} logical CPUs "
+                         f"visible); per pair {1e3 * per_pair.min():.0f}-"
+                         f"{1e3 * per_pair.max():.0f} ms; extrapolated all {P} pairs "
+                         f"{P * cpu_s / len(sample) / 3600:.1f} h; GPU values bit-identical: "
+                         f"{match}"}
 
     out = {
         "metric": METRIC,

@@ -219,12 +219,13 @@ Config config_for(const Problem &pb) {
 }
 
 // Time each candidate on a sample of the problem's items (every window, the
-// first Ps pairs — enough items to fill every resident wave twice) and keep
-// the fastest. Candidates: queries per lane {1, 2} x LDS tile {64, 128, 256}
-// frames x waves per workgroup {4, 8}.
+// first Ps pairs — enough items to give every resident wave six, so the
+// uneven tail of the last round weighs little) and keep the fastest.
+// Candidates: queries per lane {1, 2} x LDS tile {64, 128, 256} frames x
+// waves per workgroup {4, 8}.
 int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &best,
              double *best_ms) {
-    const long long target_items = 2 * max_waves();
+    const long long target_items = 6 * max_waves();
     long long Ps = std::min<long long>(pb.P, std::max<long long>(1, target_items / pb.R));
     float *d_out = (float *)workspace(kWsScreen1, (size_t)pb.R * Ps * sizeof(float));
     if (!d_out) return kHipError;
