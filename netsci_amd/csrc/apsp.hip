@@ -17,11 +17,23 @@
 // does. Out-of-range rows/columns are padded with +inf, which never wins.
 // Roofline: FP32 VALU, 2 lane-ops (add, min) per (i, j, k).
 //
-// Backtrack kernel (hedetniemi_gpu.cu:70-107): a wavefront owns one (i, j)
-// at a time; for each p from L-1 down to 0 its 64 lanes test 64 candidate q
-// at once and a ballot picks the first q != j with
+// Backtrack (hedetniemi_gpu.cu:70-107): for each (i, j) and p from L-1 down
+// to 0, take the first q != j (ascending) with
 // fl(fl(Hi[p][i][q] + A[q][k]) - len) < tol, the reference's first-hit rule.
-// A[q][k] is read from a transposed copy so each probe is coalesced.
+// Only q with A[q][k] finite or -inf can pass that test: with A[q][k] = +inf
+// the sum is +inf or NaN and the difference is +inf or NaN, never below tol;
+// NaN likewise. So the candidates of node k are the column list
+//     cand(k) = { q : A[q][k] < +inf }, ascending,
+// built once on the device (count, scan, fill; each list padded to a
+// multiple of 4 with (q = 0, +inf), which never passes), and
+//   * sparse graphs (mean list length <= kDenseDegree): one LANE per (i, j)
+//     walks cand(k) four entries at a time — at the ~11 contacts per node of
+//     a 7.5 A contact graph a step costs a few gathers instead of a scan of
+//     all N nodes;
+//   * dense graphs: one WAVE per (i, j) tests 64 q at once against a
+//     transposed copy of A (coalesced) and a ballot picks the first hit.
+// A device flag written by the scan selects the kernel, so the host never
+// waits; the other kernel exits at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -135,12 +147,137 @@ __global__ void transpose_kernel(const float *__restrict__ in, float *__restrict
     }
 }
 
-// hedetniemi_gpu.cu:70-107 with the diagonal written as paths[i][i][*] = i.
+// Mean candidate-list length above which the wave-per-(i, j) kernel runs.
+constexpr long long kDenseDegree = 128;
+
+__device__ __forceinline__ bool is_candidate(float a) { return a < __builtin_inff(); }
+
+// Candidate counts: one wave per node k scans row k of A^T (= column k of A)
+// and stores the list length rounded up to a multiple of 4.
+__global__ __launch_bounds__(256) void cand_count_kernel(const float *__restrict__ AT,
+                                                         long long *__restrict__ len4, int N) {
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (k >= N) return;
+    const float *row = AT + (long long)k * N;
+    long long cnt = 0;
+    for (int q0 = 0; q0 < N; q0 += 64) {
+        const int q = q0 + lane;
+        const bool ok = q < N && is_candidate(row[q]);
+        cnt += __popcll(__ballot(ok));
+    }
+    if (lane == 0) len4[k] = (cnt + 3) & ~3LL;
+}
+
+// Exclusive scan of the padded lengths into rows[0..N] (one workgroup of
+// 1024 threads, 1024 nodes per round) and the dense/sparse flag.
+__global__ __launch_bounds__(1024) void cand_scan_kernel(const long long *__restrict__ len4,
+                                                         long long *__restrict__ rows,
+                                                         unsigned *__restrict__ dense, int N) {
+    __shared__ long long s[1024];
+    const int t = threadIdx.x;
+    long long carry = 0;
+    for (int base = 0; base < N; base += 1024) {
+        s[t] = (base + t < N) ? len4[base + t] : 0;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const long long v = (t >= off) ? s[t - off] : 0;
+            __syncthreads();
+            s[t] += v;
+            __syncthreads();
+        }
+        if (base + t < N) rows[base + t + 1] = carry + s[t];
+        carry += s[1023];
+        __syncthreads();
+    }
+    if (t == 0) {
+        rows[0] = 0;
+        dense[0] = carry > kDenseDegree * (long long)N ? 1u : 0u;
+    }
+}
+
+// Candidate lists: one wave per node k writes the q with A[q][k] < +inf in
+// ascending order (ballot + lane rank), then the (0, +inf) padding.
+__global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict__ AT,
+                                                        const long long *__restrict__ rows,
+                                                        const unsigned *__restrict__ dense,
+                                                        int *__restrict__ cq,
+                                                        float *__restrict__ ca, int N) {
+    if (dense[0]) return;
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (k >= N) return;
+    const float *row = AT + (long long)k * N;
+    const long long beg = rows[k], end = rows[k + 1];
+    long long at = beg;
+    for (int q0 = 0; q0 < N; q0 += 64) {
+        const int q = q0 + lane;
+        const float a = q < N ? row[q] : __builtin_inff();
+        const bool ok = q < N && is_candidate(a);
+        const unsigned long long m = __ballot(ok);
+        const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (ok) {
+            cq[at + rank] = q;
+            ca[at + rank] = a;
+        }
+        at += __popcll(m);
+    }
+    for (long long e = at + lane; e < end; e += 64) {
+        cq[e] = 0;
+        ca[e] = __builtin_inff();
+    }
+}
+
+// Sparse backtrack: one lane per (i, j); i = blockIdx.x, j = blockIdx.y *
+// 256 + thread. Writes paths[i][j][0..L-1] (diagonal: i).
+__global__ __launch_bounds__(256) void backtrack_sparse_kernel(
+    const float *__restrict__ H, const float *__restrict__ Hi, const long long *__restrict__ rows,
+    const int *__restrict__ cq, const float *__restrict__ ca, const unsigned *__restrict__ dense,
+    int *__restrict__ paths, float tol, int N, int L) {
+    if (dense[0]) return;
+    const int i = blockIdx.x;
+    const int j = blockIdx.y * 256 + threadIdx.x;
+    if (j >= N) return;
+    const long long NN = (long long)N * N;
+    int *pp = paths + ((long long)i * N + j) * L;
+    if (i == j) {
+        for (int p = 0; p < L; ++p) pp[p] = i;
+        return;
+    }
+    float len = H[(long long)i * N + j];
+    int k = j;
+    for (int p = L - 1; p >= 0; --p) {
+        const float *hrow = Hi + (long long)p * NN + (long long)i * N;
+        const long long end = rows[k + 1];
+        int found = -1;
+        float af = 0.0f;
+        for (long long e = rows[k]; e < end; e += 4) {
+            const int4 q = *reinterpret_cast<const int4 *>(cq + e);
+            const float4 a = *reinterpret_cast<const float4 *>(ca + e);
+            const float h0 = hrow[q.x], h1 = hrow[q.y], h2 = hrow[q.z], h3 = hrow[q.w];
+            if (q.x != j && (h0 + a.x) - len < tol) { found = q.x; af = a.x; break; }
+            if (q.y != j && (h1 + a.y) - len < tol) { found = q.y; af = a.y; break; }
+            if (q.z != j && (h2 + a.z) - len < tol) { found = q.z; af = a.z; break; }
+            if (q.w != j && (h3 + a.w) - len < tol) { found = q.w; af = a.w; break; }
+        }
+        if (found >= 0) {
+            len -= af;
+            k = found;
+        }
+        pp[p] = k;
+    }
+}
+
+// Dense backtrack (hedetniemi_gpu.cu:70-107 with the diagonal written as
+// paths[i][i][*] = i): one wave per (i, j), 64 candidates per probe.
 __global__ __launch_bounds__(256) void backtrack_kernel(const float *__restrict__ H,
                                                         const float *__restrict__ Hi,
                                                         const float *__restrict__ AT,
+                                                        const unsigned *__restrict__ dense,
                                                         int *__restrict__ paths, float tol,
                                                         int N, int L) {
+    if (!dense[0]) return;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = blockIdx.x;
@@ -263,14 +400,35 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
         if (rc) return rc;
     }
     if (!d_paths) return kOk;
+    NSX_REQUIRE((N + 255) / 256 <= 65535, "nsx_hedetniemi: N too large for the path kernels");
     float *AT = (float *)workspace(kWsApsp0, bytes);
     if (!AT) return kHipError;
     const int tb = (N + 31) / 32;
     hipLaunchKernelGGL(transpose_kernel, dim3(tb, tb), dim3(32, 8), 0, s, d_A, AT, N);
     NSX_HIP(hipGetLastError(), "transpose launch");
-    hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, d_paths, tol,
-                       N, L);
-    NSX_HIP(hipGetLastError(), "backtrack launch");
+    // Candidate lists: rows[0..N], padded lengths[N], dense flag; then the
+    // lists (node ids, then A values), at most N * pad4(N) entries.
+    long long *rows = (long long *)workspace(kWsApspRows, (size_t)(2 * N + 2) * sizeof(long long));
+    if (!rows) return kHipError;
+    long long *len4 = rows + N + 1;
+    unsigned *dense = (unsigned *)(len4 + N);
+    const size_t cap = ((size_t)N * (size_t)(N + 3) + 3) & ~(size_t)3;
+    int *cq = (int *)workspace(kWsApspCand, cap * (sizeof(int) + sizeof(float)));
+    if (!cq) return kHipError;
+    float *ca = (float *)(cq + cap);
+    const int wb = (N + 3) / 4;
+    hipLaunchKernelGGL(cand_count_kernel, dim3(wb), dim3(256), 0, s, AT, len4, N);
+    NSX_HIP(hipGetLastError(), "candidate count launch");
+    hipLaunchKernelGGL(cand_scan_kernel, dim3(1), dim3(1024), 0, s, len4, rows, dense, N);
+    NSX_HIP(hipGetLastError(), "candidate scan launch");
+    hipLaunchKernelGGL(cand_fill_kernel, dim3(wb), dim3(256), 0, s, AT, rows, dense, cq, ca, N);
+    NSX_HIP(hipGetLastError(), "candidate fill launch");
+    hipLaunchKernelGGL(backtrack_sparse_kernel, dim3(N, (N + 255) / 256), dim3(256), 0, s, d_H,
+                       d_Hi, rows, cq, ca, dense, d_paths, tol, N, L);
+    NSX_HIP(hipGetLastError(), "sparse backtrack launch");
+    hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, dense, d_paths,
+                       tol, N, L);
+    NSX_HIP(hipGetLastError(), "dense backtrack launch");
     return kOk;
 }
 

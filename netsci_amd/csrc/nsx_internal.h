@@ -60,6 +60,8 @@ enum WorkspaceSlot : int {
     kWsApsp1 = 6,
     kWsScreen0 = 7,
     kWsScreen1 = 8,
+    kWsApspRows = 9,
+    kWsApspCand = 10,
     kWsCount = 16,
 };
 

@@ -32,8 +32,11 @@ def _gpu(A, L, tol):
     return dH.numpy(), dHi.numpy(), dP.numpy()
 
 
+# Sparse graphs take the lane-per-(i, j) candidate-list backtrack; the last
+# case (mean in-degree 270 > 128) takes the wave-per-(i, j) dense kernel.
 @pytest.mark.parametrize("N,L,density,negzero", [(37, 6, 0.15, False), (100, 12, 0.05, False),
-                                                 (130, 5, 0.3, True), (64, 1, 0.5, False)])
+                                                 (130, 5, 0.3, True), (64, 1, 0.5, False),
+                                                 (300, 4, 0.9, False)])
 def test_hedetniemi_bitexact(N, L, density, negzero):
     print(f"apsp N={N} L={L}", flush=True)
     A = _graph(N, density, seed=N + L, negzero=negzero)
@@ -41,6 +44,30 @@ def test_hedetniemi_bitexact(N, L, density, negzero):
     Ho, Hio, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
     np.testing.assert_array_equal(H.view(np.uint32), Ho.view(np.uint32))
     np.testing.assert_array_equal(Hi.view(np.uint32), Hio.view(np.uint32))
+    np.testing.assert_array_equal(P, Po)
+
+
+def test_contact_graph_paths_bitexact():
+    """The C-alpha contact-graph shape the path workload uses (-log GC on
+    pairs closer than 7.5 A, about ten candidates per node), L = hop diameter
+    + 1, against the oracle."""
+    from netsci_amd import synth
+    N, n = 400, 600
+    print(f"apsp contact graph N={N}", flush=True)
+    X, mean = synth.calpha_trajectory(N=N, n=n, seed=11)
+    pairs = synth.contact_pairs(mean)
+    # Edge weights from the GPU KSG path (bit-exact against the oracle in
+    # test_gpu_ksg.py); only the graph's shape matters here.
+    dX = abi.DeviceArray.from_numpy(X)
+    dab = abi.DeviceArray.from_numpy(pairs)
+    dmi = abi.DeviceArray((len(pairs),), np.float32)
+    abi.ksg_mi(dX, X.shape[1], n, 3, n, dab, len(pairs), 6, dmi)
+    gc = abi.gc_from_mi(dmi.numpy(), 3)
+    A = synth.contact_adjacency(gc, pairs, N)
+    L = synth.hop_diameter(A) + 1
+    H, Hi, P = _gpu(A, L, 1e-4)
+    Ho, Hio, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
+    np.testing.assert_array_equal(H.view(np.uint32), Ho.view(np.uint32))
     np.testing.assert_array_equal(P, Po)
 
 
