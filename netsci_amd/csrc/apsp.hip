@@ -17,6 +17,15 @@
 // does. Out-of-range rows/columns are padded with +inf, which never wins.
 // Roofline: FP32 VALU, 2 lane-ops (add, min) per (i, j, k).
 //
+// Sparse step (graphs whose rows of A hold at most kDenseDegree entries
+// < +inf on average, e.g. a 7.5 A contact graph): only k with
+// A[j][k] < +inf can win the fold (any other k gives +inf or NaN, which the
+// strict `<` never takes), so H_{p+1}[i][j] folds just those k, ascending,
+// with the compare-and-select — bit-identical to the dense fold. Lanes own
+// i and a wave takes one j at a time, so j's neighbour list is wave-uniform
+// and each k costs one coalesced 256-byte read of H_p[k][i0..i0+63]. Cost
+// per step: N^2 * degree / 64 such reads instead of N^3 lane-ops.
+//
 // Backtrack (hedetniemi_gpu.cu:70-107): for each (i, j) and p from L-1 down
 // to 0, take the first q != j (ascending) with
 // fl(fl(Hi[p][i][q] + A[q][k]) - len) < tol, the reference's first-hit rule.
@@ -28,8 +37,9 @@
 // multiple of 4 with (q = 0, +inf), which never passes), and
 //   * sparse graphs (mean list length <= kDenseDegree): one LANE per (i, j)
 //     walks cand(k) four entries at a time — at the ~11 contacts per node of
-//     a 7.5 A contact graph a step costs a few gathers instead of a scan of
-//     all N nodes;
+//     a 7.5 A contact graph a step costs a few reads instead of a scan of
+//     all N nodes; the workgroup's lanes share i, so the row Hi[p][i][.]
+//     they read from is staged in LDS once per step;
 //   * dense graphs: one WAVE per (i, j) tests 64 q at once against a
 //     transposed copy of A (coalesced) and a ballot picks the first hit.
 // A device flag written by the scan selects the kernel, so the host never
@@ -123,18 +133,102 @@ __device__ __forceinline__ void minplus_tile(const float *__restrict__ A,
     }
 }
 
+// Dense step; runs only when the neighbour lists of A flagged the graph
+// dense (otherwise minplus_sparse_kernel computes the same step).
 __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ A,
                                                       const float *__restrict__ Hp,
                                                       float *__restrict__ out, int N,
-                                                      const unsigned *__restrict__ negzero) {
+                                                      const unsigned *__restrict__ negzero,
+                                                      const unsigned *__restrict__ dense) {
     __shared__ __attribute__((aligned(16))) float Hs[kKStep][kTile];
     __shared__ __attribute__((aligned(16))) float As[kKStep][kTile];
+    if (!dense[0]) return;
     if (*negzero) minplus_tile<true>(A, Hp, out, N, Hs, As);
     else minplus_tile<false>(A, Hp, out, N, Hs, As);
 }
 
-__global__ void transpose_kernel(const float *__restrict__ in, float *__restrict__ out, int N) {
+// Sparse step. out[i][j] = min over k in nbr(j) (ascending) of
+// fl(A[j][k] + Hp[k][i]) with the reference's strict-`<` select from +inf,
+// where nbr(j) = { k : A[j][k] < +inf }: every other k gives +inf or NaN,
+// which the select never takes, so the result equals the dense fold bit for
+// bit (signed zeros and NaN included).
+// Workgroup = a 64 (i) x 64 (j) output tile; lanes own i, each wave takes 16
+// of the j. For one j the neighbour list is wave-uniform (scalar loads) and
+// every k reads the coalesced row segment Hp[k][i0..i0+63]. Results go
+// through an LDS tile so the output is written as row segments
+// out[i][j0..j0+63]. Workgroups g, g+8, g+16, ... share an XCD; they get the
+// same i-range, so that XCD's L2 keeps the 64-column slice of Hp they read.
+__global__ __launch_bounds__(256) void minplus_sparse_kernel(const float *__restrict__ Hp,
+                                                             const long long *__restrict__ rows,
+                                                             const int *__restrict__ lst,
+                                                             const unsigned *__restrict__ dense,
+                                                             float *__restrict__ out, int N,
+                                                             int nI, int nJ) {
+    __shared__ float T[64][65];
+    if (dense[0]) return;
+    const int g = blockIdx.x;
+    const int s = g >> 3;
+    const int iI = (g & 7) + 8 * (s / nJ);
+    const int jJ = s % nJ;
+    if (iI >= nI) return;
+    const int i0 = iI * 64, j0 = jJ * 64;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = i0 + lane;
+    const float *col = Hp + (i < N ? i : 0);  // col[k * N] = Hp[k][i]
+    for (int jj = w; jj < 64; jj += 4) {
+        const int j = j0 + jj;
+        float c = __builtin_inff();
+        if (j < N) {
+            const long long end = rows[j + 1];
+            long long e = rows[j];
+            // 16 entries per round: the list loads and the 16 row reads are
+            // all in flight before the first fold.
+            for (; e + 16 <= end; e += 16) {
+                int kq[16];
+                float ka[16], v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    kq[u] = lst[2 * e + 8 * (u >> 2) + (u & 3)];
+                    ka[u] = __int_as_float(lst[2 * e + 8 * (u >> 2) + 4 + (u & 3)]);
+                }
+#pragma unroll
+                for (int u = 0; u < 16; ++u) v[u] = col[(long long)kq[u] * N];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const float t = ka[u] + v[u];
+                    c = (t < c) ? t : c;
+                }
+            }
+            for (; e < end; e += 4) {
+                const int4 k4 = *reinterpret_cast<const int4 *>(lst + 2 * e);
+                const float4 a4 = *reinterpret_cast<const float4 *>(lst + 2 * e + 4);
+                const float v0 = col[(long long)k4.x * N], v1 = col[(long long)k4.y * N];
+                const float v2 = col[(long long)k4.z * N], v3 = col[(long long)k4.w * N];
+                float t = a4.x + v0;
+                c = (t < c) ? t : c;
+                t = a4.y + v1;
+                c = (t < c) ? t : c;
+                t = a4.z + v2;
+                c = (t < c) ? t : c;
+                t = a4.w + v3;
+                c = (t < c) ? t : c;
+            }
+        }
+        T[lane][jj] = c;
+    }
+    __syncthreads();
+    for (int r = w; r < 64; r += 4) {
+        const int ii = i0 + r, j = j0 + lane;
+        if (ii < N && j < N) out[(long long)ii * N + j] = T[r][lane];
+    }
+}
+
+// out = in^T; skipped entirely when `skip` is non-null and set.
+__global__ void transpose_kernel(const float *__restrict__ in, float *__restrict__ out, int N,
+                                 const unsigned *__restrict__ skip) {
     __shared__ float t[32][33];
+    if (skip && skip[0]) return;
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
         const int row = by + r, col = bx + threadIdx.x;
@@ -173,7 +267,8 @@ __global__ __launch_bounds__(256) void cand_count_kernel(const float *__restrict
 // 1024 threads, 1024 nodes per round) and the dense/sparse flag.
 __global__ __launch_bounds__(1024) void cand_scan_kernel(const long long *__restrict__ len4,
                                                          long long *__restrict__ rows,
-                                                         unsigned *__restrict__ dense, int N) {
+                                                         unsigned *__restrict__ dense, int N,
+                                                         int allow_sparse) {
     __shared__ long long s[1024];
     const int t = threadIdx.x;
     long long carry = 0;
@@ -192,24 +287,33 @@ __global__ __launch_bounds__(1024) void cand_scan_kernel(const long long *__rest
     }
     if (t == 0) {
         rows[0] = 0;
-        dense[0] = carry > kDenseDegree * (long long)N ? 1u : 0u;
+        dense[0] = (!allow_sparse || carry > kDenseDegree * (long long)N) ? 1u : 0u;
     }
 }
 
-// Candidate lists: one wave per node k writes the q with A[q][k] < +inf in
-// ascending order (ballot + lane rank), then the (0, +inf) padding.
-__global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict__ AT,
+// Candidate lists: one wave per node k (row k of M) writes the columns c
+// with M[k][c] < +inf in ascending order (ballot + lane rank), then the
+// (0, +inf) padding. Entries are stored in groups of four as 32 bytes,
+// {c0, c1, c2, c3, a0, a1, a2, a3}, so entry e's column is lst[2e - e%4 ...]:
+// group e/4 starts at int offset 8 * (e / 4) = 2e - 2 (e % 4).
+__device__ __forceinline__ void put_entry(int *__restrict__ lst, long long e, int c, float a) {
+    const long long g = (e >> 2) << 3;
+    const int slot = (int)(e & 3);
+    lst[g + slot] = c;
+    lst[g + 4 + slot] = __float_as_int(a);
+}
+
+__global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict__ M,
                                                         const long long *__restrict__ rows,
                                                         const unsigned *__restrict__ dense,
-                                                        int *__restrict__ cq,
-                                                        float *__restrict__ ca, int N) {
+                                                        int *__restrict__ lst, int N) {
     if (dense[0]) return;
     const int lane = threadIdx.x & 63;
     const int k = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (k >= N) return;
-    const float *row = AT + (long long)k * N;
-    const long long beg = rows[k], end = rows[k + 1];
-    long long at = beg;
+    const float *row = M + (long long)k * N;
+    const long long end = rows[k + 1];
+    long long at = rows[k];
     for (int q0 = 0; q0 < N; q0 += 64) {
         const int q = q0 + lane;
         const float a = q < N ? row[q] : __builtin_inff();
@@ -217,45 +321,64 @@ __global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict_
         const unsigned long long m = __ballot(ok);
         const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (ok) {
-            cq[at + rank] = q;
-            ca[at + rank] = a;
-        }
+        if (ok) put_entry(lst, at + rank, q, a);
         at += __popcll(m);
     }
-    for (long long e = at + lane; e < end; e += 64) {
-        cq[e] = 0;
-        ca[e] = __builtin_inff();
-    }
+    for (long long e = at + lane; e < end; e += 64) put_entry(lst, e, 0, __builtin_inff());
 }
 
-// Sparse backtrack: one lane per (i, j); i = blockIdx.x, j = blockIdx.y *
-// 256 + thread. Writes paths[i][j][0..L-1] (diagonal: i).
-__global__ __launch_bounds__(256) void backtrack_sparse_kernel(
+// A node's list span [rows[k], rows[k + 1]) in one 16-byte load.
+struct __attribute__((aligned(8))) Span {
+    long long beg, end;
+};
+
+// Sparse backtrack: one lane per (i, j); a workgroup of 1024 lanes takes
+// one i and 1024 consecutive j. Every step p all its lanes read from the row
+// Hi[p][i][.], so the workgroup stages that row in LDS (kLdsRow; N <=
+// kLdsRowMax) and the per-candidate reads become LDS reads. Workgroups
+// g, g+8, ... share an XCD and get the same i, so the row comes from that
+// XCD's L2 for all but the first of them. Writes paths[i][j][0..L-1]
+// (diagonal: i).
+constexpr int kBtThreads = 1024;
+constexpr int kLdsRowMax = 16384;
+
+template <bool kLdsRow>
+__global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
     const float *__restrict__ H, const float *__restrict__ Hi, const long long *__restrict__ rows,
-    const int *__restrict__ cq, const float *__restrict__ ca, const unsigned *__restrict__ dense,
-    int *__restrict__ paths, float tol, int N, int L) {
+    const int *__restrict__ lst, const unsigned *__restrict__ dense, int *__restrict__ paths,
+    float tol, int N, int L, int nC) {
+    extern __shared__ float hrow_lds[];
     if (dense[0]) return;
-    const int i = blockIdx.x;
-    const int j = blockIdx.y * 256 + threadIdx.x;
-    if (j >= N) return;
+    const int g = blockIdx.x;
+    const int s = g >> 3;
+    const int i = (g & 7) + 8 * (s / nC);
+    if (i >= N) return;
+    const int j = (s % nC) * kBtThreads + threadIdx.x;
     const long long NN = (long long)N * N;
-    int *pp = paths + ((long long)i * N + j) * L;
-    if (i == j) {
+    const bool active = j < N && j != i;
+    int *pp = paths + ((long long)i * N + (j < N ? j : 0)) * L;
+    if (j < N && j == i)
         for (int p = 0; p < L; ++p) pp[p] = i;
-        return;
-    }
-    float len = H[(long long)i * N + j];
+    float len = active ? H[(long long)i * N + j] : 0.0f;
     int k = j;
     for (int p = L - 1; p >= 0; --p) {
         const float *hrow = Hi + (long long)p * NN + (long long)i * N;
-        const long long end = rows[k + 1];
+        if (kLdsRow) {
+            __syncthreads();  // the previous step's reads are done
+            for (int t = threadIdx.x; t < N; t += kBtThreads) hrow_lds[t] = hrow[t];
+            __syncthreads();
+        }
+        if (!active) continue;
+        const Span sp = *reinterpret_cast<const Span *>(rows + k);
         int found = -1;
         float af = 0.0f;
-        for (long long e = rows[k]; e < end; e += 4) {
-            const int4 q = *reinterpret_cast<const int4 *>(cq + e);
-            const float4 a = *reinterpret_cast<const float4 *>(ca + e);
-            const float h0 = hrow[q.x], h1 = hrow[q.y], h2 = hrow[q.z], h3 = hrow[q.w];
+        for (long long e = sp.beg; e < sp.end; e += 4) {
+            const int4 q = *reinterpret_cast<const int4 *>(lst + 2 * e);
+            const float4 a = *reinterpret_cast<const float4 *>(lst + 2 * e + 4);
+            const float h0 = kLdsRow ? hrow_lds[q.x] : hrow[q.x];
+            const float h1 = kLdsRow ? hrow_lds[q.y] : hrow[q.y];
+            const float h2 = kLdsRow ? hrow_lds[q.z] : hrow[q.z];
+            const float h3 = kLdsRow ? hrow_lds[q.w] : hrow[q.w];
             if (q.x != j && (h0 + a.x) - len < tol) { found = q.x; af = a.x; break; }
             if (q.y != j && (h1 + a.y) - len < tol) { found = q.y; af = a.y; break; }
             if (q.z != j && (h2 + a.z) - len < tol) { found = q.z; af = a.z; break; }
@@ -327,10 +450,48 @@ __global__ void corr_to_adj_kernel(const float *__restrict__ C, float *__restric
     }
 }
 
-int launch_minplus(const float *A, const float *Hp, float *out, int N, const unsigned *flag,
-                   hipStream_t s) {
+// Per-row candidate lists of M (row r: the columns c with M[r][c] < +inf,
+// ascending, padded to a multiple of 4) and the dense flag, in the given
+// workspace slots. allow_sparse = false forces the dense flag.
+struct Lists {
+    long long *rows = nullptr;
+    int *lst = nullptr;  // groups of 4 entries: 4 node ids, then 4 weights
+    unsigned *dense = nullptr;
+};
+
+int build_lists(const float *M, int N, int slot_rows, int slot_cand, bool allow_sparse,
+                hipStream_t s, Lists &l) {
+    l.rows = (long long *)workspace(slot_rows, (size_t)(2 * N + 2) * sizeof(long long));
+    if (!l.rows) return kHipError;
+    long long *len4 = l.rows + N + 1;
+    l.dense = (unsigned *)(len4 + N);
+    const size_t cap = ((size_t)N * (size_t)(N + 3) + 3) & ~(size_t)3;
+    l.lst = (int *)workspace(slot_cand, cap * (sizeof(int) + sizeof(float)));
+    if (!l.lst) return kHipError;
+    const int wb = (N + 3) / 4;
+    hipLaunchKernelGGL(cand_count_kernel, dim3(wb), dim3(256), 0, s, M, len4, N);
+    NSX_HIP(hipGetLastError(), "candidate count launch");
+    hipLaunchKernelGGL(cand_scan_kernel, dim3(1), dim3(1024), 0, s, len4, l.rows, l.dense, N,
+                       allow_sparse ? 1 : 0);
+    NSX_HIP(hipGetLastError(), "candidate scan launch");
+    hipLaunchKernelGGL(cand_fill_kernel, dim3(wb), dim3(256), 0, s, M, l.rows, l.dense, l.lst, N);
+    NSX_HIP(hipGetLastError(), "candidate fill launch");
+    return kOk;
+}
+
+// One Jacobi step out = step(Hp). Both variants are launched; the dense
+// flag of A's neighbour lists lets exactly one of them do the work, so the
+// host never reads the flag back.
+int minplus_step(const float *A, const float *Hp, float *out, int N, const unsigned *negzero,
+                 const Lists &la, hipStream_t s) {
+    const int nT = (N + 63) / 64;
+    const int groups = 8 * ((nT + 7) / 8) * nT;
+    hipLaunchKernelGGL(minplus_sparse_kernel, dim3(groups), dim3(256), 0, s, Hp, la.rows, la.lst,
+                       la.dense, out, N, nT, nT);
+    NSX_HIP(hipGetLastError(), "sparse min-plus launch");
     const int tiles = (N + kTile - 1) / kTile;
-    hipLaunchKernelGGL(minplus_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, Hp, out, N, flag);
+    hipLaunchKernelGGL(minplus_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, Hp, out, N,
+                       negzero, (const unsigned *)la.dense);
     NSX_HIP(hipGetLastError(), "min-plus launch");
     return kOk;
 }
@@ -348,6 +509,14 @@ int negzero_flag(const float *A, int N, unsigned **flag, hipStream_t s) {
     return kOk;
 }
 
+// Shared set-up of both entry points: the negative-zero flag (dense step)
+// and A's neighbour lists with the dense/sparse flag.
+int step_setup(const float *A, int N, hipStream_t s, unsigned **negzero, Lists &la) {
+    int rc = negzero_flag(A, N, negzero, s);
+    if (rc) return rc;
+    return build_lists(A, N, kWsApspRowsA, kWsApspCandA, true, s, la);
+}
+
 }  // namespace
 }  // namespace nsx
 
@@ -362,8 +531,9 @@ int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *str
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     const size_t bytes = (size_t)N * N * sizeof(float);
-    unsigned *flag = nullptr;
-    rc = negzero_flag(d_A, N, &flag, s);
+    unsigned *negzero = nullptr;
+    Lists la;
+    rc = step_setup(d_A, N, s, &negzero, la);
     if (rc) return rc;
     float *tmp = (float *)workspace(kWsApsp0, bytes);
     if (!tmp) return kHipError;
@@ -372,7 +542,7 @@ int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *str
     int cur = (L % 2 == 0) ? 0 : 1;
     NSX_HIP(hipMemcpyAsync(bufs[cur], d_A, bytes, hipMemcpyDeviceToDevice, s), "H0 = A");
     for (int p = 0; p < L; ++p) {
-        rc = launch_minplus(d_A, bufs[cur], bufs[cur ^ 1], N, flag, s);
+        rc = minplus_step(d_A, bufs[cur], bufs[cur ^ 1], N, negzero, la, s);
         if (rc) return rc;
         cur ^= 1;
     }
@@ -390,44 +560,40 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     hipStream_t s = (hipStream_t)stream;
     const long long NN = (long long)N * N;
     const size_t bytes = (size_t)NN * sizeof(float);
-    unsigned *flag = nullptr;
-    rc = negzero_flag(d_A, N, &flag, s);
+    unsigned *negzero = nullptr;
+    Lists la;
+    rc = step_setup(d_A, N, s, &negzero, la);
     if (rc) return rc;
     NSX_HIP(hipMemcpyAsync(d_Hi, d_A, bytes, hipMemcpyDeviceToDevice, s), "Hi[0] = A");
     for (int p = 0; p < L; ++p) {
         float *out = (p + 1 < L) ? d_Hi + (size_t)(p + 1) * NN : d_H;
-        rc = launch_minplus(d_A, d_Hi + (size_t)p * NN, out, N, flag, s);
+        rc = minplus_step(d_A, d_Hi + (size_t)p * NN, out, N, negzero, la, s);
         if (rc) return rc;
     }
     if (!d_paths) return kOk;
-    NSX_REQUIRE((N + 255) / 256 <= 65535, "nsx_hedetniemi: N too large for the path kernels");
+    // Backtrack candidates of node k are the q with A[q][k] < +inf: the
+    // per-row lists of A^T.
     float *AT = (float *)workspace(kWsApsp0, bytes);
     if (!AT) return kHipError;
     const int tb = (N + 31) / 32;
-    hipLaunchKernelGGL(transpose_kernel, dim3(tb, tb), dim3(32, 8), 0, s, d_A, AT, N);
+    hipLaunchKernelGGL(transpose_kernel, dim3(tb, tb), dim3(32, 8), 0, s, d_A, AT, N,
+                       (const unsigned *)nullptr);
     NSX_HIP(hipGetLastError(), "transpose launch");
-    // Candidate lists: rows[0..N], padded lengths[N], dense flag; then the
-    // lists (node ids, then A values), at most N * pad4(N) entries.
-    long long *rows = (long long *)workspace(kWsApspRows, (size_t)(2 * N + 2) * sizeof(long long));
-    if (!rows) return kHipError;
-    long long *len4 = rows + N + 1;
-    unsigned *dense = (unsigned *)(len4 + N);
-    const size_t cap = ((size_t)N * (size_t)(N + 3) + 3) & ~(size_t)3;
-    int *cq = (int *)workspace(kWsApspCand, cap * (sizeof(int) + sizeof(float)));
-    if (!cq) return kHipError;
-    float *ca = (float *)(cq + cap);
-    const int wb = (N + 3) / 4;
-    hipLaunchKernelGGL(cand_count_kernel, dim3(wb), dim3(256), 0, s, AT, len4, N);
-    NSX_HIP(hipGetLastError(), "candidate count launch");
-    hipLaunchKernelGGL(cand_scan_kernel, dim3(1), dim3(1024), 0, s, len4, rows, dense, N);
-    NSX_HIP(hipGetLastError(), "candidate scan launch");
-    hipLaunchKernelGGL(cand_fill_kernel, dim3(wb), dim3(256), 0, s, AT, rows, dense, cq, ca, N);
-    NSX_HIP(hipGetLastError(), "candidate fill launch");
-    hipLaunchKernelGGL(backtrack_sparse_kernel, dim3(N, (N + 255) / 256), dim3(256), 0, s, d_H,
-                       d_Hi, rows, cq, ca, dense, d_paths, tol, N, L);
+    Lists lt;
+    rc = build_lists(AT, N, kWsApspRows, kWsApspCand, true, s, lt);
+    if (rc) return rc;
+    const int nC = (N + kBtThreads - 1) / kBtThreads;
+    const int bt_groups = 8 * ((N + 7) / 8) * nC;
+    if (N <= kLdsRowMax)
+        hipLaunchKernelGGL(backtrack_sparse_kernel<true>, dim3(bt_groups), dim3(kBtThreads),
+                           (size_t)N * sizeof(float), s, d_H, d_Hi, lt.rows, lt.lst, lt.dense,
+                           d_paths, tol, N, L, nC);
+    else
+        hipLaunchKernelGGL(backtrack_sparse_kernel<false>, dim3(bt_groups), dim3(kBtThreads), 0,
+                           s, d_H, d_Hi, lt.rows, lt.lst, lt.dense, d_paths, tol, N, L, nC);
     NSX_HIP(hipGetLastError(), "sparse backtrack launch");
-    hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, dense, d_paths,
-                       tol, N, L);
+    hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, lt.dense,
+                       d_paths, tol, N, L);
     NSX_HIP(hipGetLastError(), "dense backtrack launch");
     return kOk;
 }

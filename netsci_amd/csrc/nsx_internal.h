@@ -62,6 +62,9 @@ enum WorkspaceSlot : int {
     kWsScreen1 = 8,
     kWsApspRows = 9,
     kWsApspCand = 10,
+    kWsApspRowsA = 11,
+    kWsApspCandA = 12,
+    kWsApspG = 13,
     kWsCount = 16,
 };
 

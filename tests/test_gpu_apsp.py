@@ -47,6 +47,25 @@ def test_hedetniemi_bitexact(N, L, density, negzero):
     np.testing.assert_array_equal(P, Po)
 
 
+def test_sparse_special_values_bitexact():
+    """NaN and -0.0 weights on a sparse graph: the neighbour lists drop
+    exactly the entries the reference's tests can never take (+inf, NaN), and
+    the sparse step keeps the strict-`<` first minimum between -0.0 and +0.0,
+    so distances and paths still match the oracle bit for bit."""
+    N, L = 90, 6
+    print(f"apsp special values N={N}", flush=True)
+    rng = np.random.default_rng(21)
+    A = _graph(N, 0.08, seed=21)
+    A[rng.random((N, N)) < 0.01] = np.nan
+    A[rng.random((N, N)) < 0.01] = -0.0
+    np.fill_diagonal(A, 0.0)
+    H, Hi, P = _gpu(A, L, 1e-4)
+    Ho, Hio, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
+    np.testing.assert_array_equal(H.view(np.uint32), Ho.view(np.uint32))
+    np.testing.assert_array_equal(Hi.view(np.uint32), Hio.view(np.uint32))
+    np.testing.assert_array_equal(P, Po)
+
+
 def test_contact_graph_paths_bitexact():
     """The C-alpha contact-graph shape the path workload uses (-log GC on
     pairs closer than 7.5 A, about ten candidates per node), L = hop diameter
