@@ -53,6 +53,9 @@ def parse():
                          "frame windows of a 512-node, 4000-frame trajectory, one graph-captured "
                          "batched launch vs one launch per replicate; apsp: Hedetniemi on the "
                          "-log(GC) contact graph of the gc workload")
+    ap.add_argument("--pass2", choices=["default", "plain", "symmetric"], default="default",
+                    help="KSG pass 2 for the gc workload: the library default, each ordered "
+                         "frame pair, or each unordered pair once (both frames credited)")
     ap.add_argument("--valu-probe", action="store_true",
                     help="only time the VALU throughput probe (v_sub, v_max3, v_fma, KSG mix)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -118,6 +121,8 @@ def main():
     if args.workload == "screened":
         return bench_screened(args, abi, synth)
     dev_name, cus, _ = abi.device_info()
+    if args.pass2 != "default":
+        abi.set_config(2, 128, 4, 0, 1 if args.pass2 == "symmetric" else 0)
     N, n, d, k = args.nodes, args.frames, 3, args.k
     X, _ = synth.calpha_trajectory(N=N, n=n, seed=1)
     ab = synth.all_pairs(N)

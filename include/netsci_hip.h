@@ -99,10 +99,12 @@ int nsx_ksg_last_tie_frames(long long *count);
 
 /* Launch shape for the eager KSG calls above: queries per lane (1 or 2),
  * LDS frame tile (64, 128 or 256), waves per workgroup (1..8), workgroups per
- * CU (0 = as many as fit). qpl = 0 clears the override (tuned or default
- * shapes apply again). Results do not depend on the shape. No reference
- * counterpart (its launch shape is fixed, mutual_information_gpu.cu:1055-1062). */
-int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu);
+ * CU (0 = as many as fit), and symmetric = 1 for the pass-2 variant that
+ * evaluates each unordered frame pair once and credits both frames (ballot +
+ * popcount). qpl = 0 clears the override (tuned or default shapes apply
+ * again). Results do not depend on the shape. No reference counterpart (its
+ * launch shape is fixed, mutual_information_gpu.cu:1055-1062). */
+int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symmetric);
 
 /* ---- batched plans: one hipGraph launch per replay ----------------------
  * A plan fixes X, the windows, the pairs and the output of
@@ -123,7 +125,7 @@ int nsx_ksg_plan_launch(nsx_ksg_plan plan, void *stream);
 /* Tie frames of the last replay (synchronises the device). */
 int nsx_ksg_plan_tie_frames(nsx_ksg_plan plan, long long *count);
 int nsx_ksg_plan_config(nsx_ksg_plan plan, int *qpl, int *chunk, int *wpb,
-                        long long *blocks, double *tune_ms);
+                        long long *blocks, double *tune_ms, int *symmetric);
 int nsx_ksg_plan_destroy(nsx_ksg_plan plan);
 
 /* GC transform of generalized_correlation_{cpu,gpu}.cpp:8-33 on host data:

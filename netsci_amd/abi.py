@@ -38,12 +38,12 @@ SIGNATURES = {
     "nsx_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP, _VP],
     "nsx_ksg_mi_windows": [_VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I, _VP, _VP],
     "nsx_ksg_last_tie_frames": [_LLP],
-    "nsx_ksg_set_config": [_I, _I, _I, _I],
+    "nsx_ksg_set_config": [_I, _I, _I, _I, _I],
     "nsx_ksg_plan_create": [ctypes.POINTER(_VP), _VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I,
                             _VP, _I],
     "nsx_ksg_plan_launch": [_VP, _VP],
     "nsx_ksg_plan_tie_frames": [_VP, _LLP],
-    "nsx_ksg_plan_config": [_VP, _IP, _IP, _IP, _LLP, ctypes.POINTER(ctypes.c_double)],
+    "nsx_ksg_plan_config": [_VP, _IP, _IP, _IP, _LLP, ctypes.POINTER(ctypes.c_double), _IP],
     "nsx_ksg_plan_destroy": [_VP],
     "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
     "nsx_psi_host": [_VP, _I],
@@ -162,9 +162,12 @@ def ksg_mi_windows(dX, ld, dstride, d, windows, dab, P, k, dmi, stream=None):
                                    dab.ptr, P, k, dmi.ptr, stream), "nsx_ksg_mi_windows")
 
 
-def set_config(qpl=0, chunk=128, wpb=4, blocks_per_cu=0):
-    """Force the eager KSG launch shape (qpl=0 restores tuned/default shapes)."""
-    check(lib().nsx_ksg_set_config(qpl, chunk, wpb, blocks_per_cu), "set config")
+def set_config(qpl=0, chunk=128, wpb=4, blocks_per_cu=0, symmetric=0):
+    """Force the eager KSG launch shape (qpl=0 restores tuned/default shapes);
+    symmetric=1 selects the pass 2 that evaluates each unordered frame pair
+    once."""
+    check(lib().nsx_ksg_set_config(qpl, chunk, wpb, blocks_per_cu, int(symmetric)),
+          "set config")
 
 
 class KsgPlan:
@@ -189,13 +192,14 @@ class KsgPlan:
         return c.value
 
     def config(self):
-        q, c, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        q, c, w, sy = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         b, t = ctypes.c_longlong(), ctypes.c_double()
         check(lib().nsx_ksg_plan_config(self.ptr, ctypes.byref(q), ctypes.byref(c),
-                                        ctypes.byref(w), ctypes.byref(b), ctypes.byref(t)),
+                                        ctypes.byref(w), ctypes.byref(b), ctypes.byref(t),
+                                        ctypes.byref(sy)),
               "plan config")
         return {"queries_per_lane": q.value, "frame_tile": c.value, "waves_per_workgroup": w.value,
-                "workgroups": b.value, "tune_ms": t.value}
+                "symmetric_pass2": sy.value, "workgroups": b.value, "tune_ms": t.value}
 
     def close(self):
         if self.ptr is not None and self.ptr.value:

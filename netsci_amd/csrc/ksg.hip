@@ -37,13 +37,14 @@ struct Config {
     int chunk = 128; // frames per LDS tile (64, 128, 256)
     int wpb = 4;     // waves per workgroup (2, 4, 8)
     int occ = 0;     // workgroups per CU (0: as many as fit)
+    int sym = 0;     // 1: symmetric pass 2 (each unordered frame pair once)
 };
 
-KernelFn kernel_for(int d, int k, int qpl) {
+KernelFn kernel_for(int d, int k, int qpl, bool sym) {
     switch (d) {
-        case 1: return ksg::kernel_d1(k, qpl);
-        case 2: return ksg::kernel_d2(k, qpl);
-        case 3: return ksg::kernel_d3(k, qpl);
+        case 1: return ksg::kernel_d1(k, qpl, sym);
+        case 2: return ksg::kernel_d2(k, qpl, sym);
+        case 3: return ksg::kernel_d3(k, qpl, sym);
         default: return nullptr;
     }
 }
@@ -111,12 +112,12 @@ struct Shape {
 };
 
 int shape_for(const Problem &pb, const Config &c, long long items, Shape &s) {
-    s.fn = kernel_for(pb.d, pb.k, c.qpl);
+    s.fn = kernel_for(pb.d, pb.k, c.qpl, c.sym != 0);
     NSX_REQUIRE(s.fn, "ksg: no kernel for (d, k, queries per lane)");
     NSX_REQUIRE((c.chunk == 64 || c.chunk == 128 || c.chunk == 256) && c.wpb >= 1 &&
                     c.wpb <= ksg::kMaxWavesPerBlock,
                 "ksg: bad launch configuration");
-    s.lds = ksg::lds_bytes(pb.d, c.qpl, c.chunk, c.wpb);
+    s.lds = ksg::lds_bytes(pb.d, c.qpl, c.wpb, c.sym != 0);
     int per_cu = 0;
     NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s.fn,
                                                          kWave * c.wpb, s.lds),
@@ -156,7 +157,8 @@ Args make_args(const Problem &pb, const Config &c, const Buffers &b, float *mi_o
     a.mi_out = mi_out;
     a.counts_out = counts_out;
     a.scratch = b.scratch;
-    a.scratch_stride = pb.nmax;
+    a.scratch_stride = ksg::scratch_records(pb.nmax, c.sym != 0);
+    a.scratch_n = pb.nmax;
     a.stats = b.stats;
     a.chunk = c.chunk;
     a.wpb = c.wpb;
@@ -195,10 +197,15 @@ void load_tune_cache() {
     g_tune_loaded = true;
     const char *path = std::getenv("NSX_TUNE_CACHE");
     if (!path) return;
+    // One line per key: qpl chunk wpb occ sym.
     std::ifstream f(path);
-    std::string key;
-    Config c;
-    while (f >> key >> c.qpl >> c.chunk >> c.wpb >> c.occ) g_tuned[key] = c;
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream is(line);
+        std::string key;
+        Config c;
+        if (is >> key >> c.qpl >> c.chunk >> c.wpb >> c.occ >> c.sym) g_tuned[key] = c;
+    }
 }
 
 void store_tune(const std::string &key, const Config &c) {
@@ -206,7 +213,8 @@ void store_tune(const std::string &key, const Config &c) {
     const char *path = std::getenv("NSX_TUNE_CACHE");
     if (!path) return;
     std::ofstream f(path, std::ios::app);
-    f << key << " " << c.qpl << " " << c.chunk << " " << c.wpb << " " << c.occ << "\n";
+    f << key << " " << c.qpl << " " << c.chunk << " " << c.wpb << " " << c.occ << " " << c.sym
+      << "\n";
 }
 
 Config config_for(const Problem &pb) {
@@ -221,8 +229,8 @@ Config config_for(const Problem &pb) {
 // Time each candidate on a sample of the problem's items (every window, the
 // first Ps pairs — enough items to give every resident wave six, so the
 // uneven tail of the last round weighs little) and keep the fastest.
-// Candidates: queries per lane {1, 2} x LDS tile {64, 128, 256} frames x
-// waves per workgroup {4, 8}.
+// Candidates: pass 2 plain or symmetric x queries per lane {1, 2} x LDS
+// tile {64, 128, 256} frames x waves per workgroup {4, 8}.
 int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &best,
              double *best_ms) {
     const long long target_items = 6 * max_waves();
@@ -233,6 +241,7 @@ int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &be
     NSX_HIP(hipEventCreate(&e0), "tune event");
     NSX_HIP(hipEventCreate(&e1), "tune event");
     double best_t = 1e300;
+    for (int sym : {0, 1})
     for (int qpl : {1, 2})
         for (int chunk : {64, 128, 256})
             for (int wpb : {4, 8}) {
@@ -240,6 +249,7 @@ int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &be
                 c.qpl = qpl;
                 c.chunk = chunk;
                 c.wpb = wpb;
+                c.sym = sym;
                 Shape s;
                 int rc = shape_for(pb, c, (long long)pb.R * Ps, s);
                 if (rc) return rc;
@@ -283,8 +293,9 @@ int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t str
     Buffers b;
     b.psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));
     b.stats = (unsigned long long *)workspace(kWsCounter, 64);
-    b.scratch = (KeyIdx *)workspace(kWsTieScratch,
-                                    (size_t)s.blocks * c.wpb * pb.nmax * sizeof(KeyIdx));
+    b.scratch = (KeyIdx *)workspace(
+        kWsTieScratch,
+        (size_t)s.blocks * c.wpb * ksg::scratch_records(pb.nmax, c.sym != 0) * sizeof(KeyIdx));
     b.win = (int *)workspace(kWsWindows, pb.host_win.size() * sizeof(int));
     if (!b.psi || !b.stats || !b.scratch || !b.win) return kHipError;
     NSX_HIP(hipMemcpyAsync(b.psi, psi.data(), psi.size() * sizeof(float),
@@ -365,7 +376,7 @@ int nsx_ksg_last_tie_frames(long long *count) {
     return kOk;
 }
 
-int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu) {
+int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symmetric) {
     clear_error();
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     if (qpl == 0) {
@@ -373,9 +384,10 @@ int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu) {
         return kOk;
     }
     NSX_REQUIRE((qpl == 1 || qpl == 2) && (chunk == 64 || chunk == 128 || chunk == 256) &&
-                    wpb >= 1 && wpb <= ksg::kMaxWavesPerBlock && blocks_per_cu >= 0,
+                    wpb >= 1 && wpb <= ksg::kMaxWavesPerBlock && blocks_per_cu >= 0 &&
+                    (symmetric == 0 || symmetric == 1),
                 "nsx_ksg_set_config: bad configuration");
-    g_override = Config{qpl, chunk, wpb, blocks_per_cu};
+    g_override = Config{qpl, chunk, wpb, blocks_per_cu, symmetric};
     g_has_override = true;
     return kOk;
 }
@@ -400,7 +412,8 @@ int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long
     // Plan-owned buffers (allocated once, reused by every replay).
     std::vector<float> psi(pb.nmax + 1);
     psi_table(psi.data(), pb.nmax);
-    const size_t scratch_bytes = (size_t)max_waves() * pb.nmax * sizeof(KeyIdx);
+    const size_t scratch_bytes =
+        (size_t)max_waves() * ksg::scratch_records(pb.nmax, true) * sizeof(KeyIdx);
     hipError_t e = hipMalloc(&p->buf.psi, psi.size() * sizeof(float));
     if (e == hipSuccess) e = hipMalloc(&p->buf.win, pb.host_win.size() * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&p->buf.scratch, scratch_bytes);
@@ -437,7 +450,9 @@ int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long
         p->cfg = config_for(pb);
     }
     rc = shape_for(pb, p->cfg, (long long)pb.R * pb.P, p->shape);
-    if (!rc && (size_t)p->shape.blocks * p->cfg.wpb * pb.nmax * sizeof(KeyIdx) > scratch_bytes) {
+    if (!rc && (size_t)p->shape.blocks * p->cfg.wpb *
+                       ksg::scratch_records(pb.nmax, p->cfg.sym != 0) * sizeof(KeyIdx) >
+                   scratch_bytes) {
         set_error("nsx_ksg_plan_create: tie scratch too small for the chosen shape");
         rc = kInternal;
     }
@@ -494,9 +509,10 @@ int nsx_ksg_plan_tie_frames(nsx_ksg_plan plan, long long *count) {
 }
 
 int nsx_ksg_plan_config(nsx_ksg_plan plan, int *qpl, int *chunk, int *wpb, long long *blocks,
-                        double *tune_ms) {
+                        double *tune_ms, int *symmetric) {
     clear_error();
     NSX_REQUIRE(plan, "nsx_ksg_plan_config: no plan");
+    if (symmetric) *symmetric = plan->cfg.sym;
     if (qpl) *qpl = plan->cfg.qpl;
     if (chunk) *chunk = plan->cfg.chunk;
     if (wpb) *wpb = plan->cfg.wpb;

@@ -5,7 +5,7 @@
 namespace nsx {
 namespace ksg {
 
-KernelFn kernel_d2(int k, int qpl) { return kernel_lookup<2>(k, qpl); }
+KernelFn kernel_d2(int k, int qpl, bool sym) { return kernel_lookup<2>(k, qpl, sym); }
 
 }  // namespace ksg
 }  // namespace nsx

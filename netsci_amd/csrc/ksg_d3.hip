@@ -5,7 +5,7 @@
 namespace nsx {
 namespace ksg {
 
-KernelFn kernel_d3(int k, int qpl) { return kernel_lookup<3>(k, qpl); }
+KernelFn kernel_d3(int k, int qpl, bool sym) { return kernel_lookup<3>(k, qpl, sym); }
 
 }  // namespace ksg
 }  // namespace nsx

@@ -59,6 +59,7 @@ struct Args {
     int *counts_out;     // nullable: [p][frame][2] (single window only)
     KeyIdx *scratch;     // per wave: scratch_stride records
     long long scratch_stride;
+    long long scratch_n; // records per scratch region (max window length)
     unsigned long long *stats;  // [0] tie frames
     int chunk;           // frames per LDS tile: 64, 128 or 256
     int wpb;             // waves per workgroup
@@ -66,16 +67,25 @@ struct Args {
 
 using KernelFn = void (*)(Args);
 
-// Dynamic LDS bytes a launch needs.
-inline size_t lds_bytes(int d, int qpl, int chunk, int wpb) {
-    (void)chunk;
-    return (size_t)wpb * (size_t)(2 * d * kMaxChunk + kWave * qpl) * sizeof(float);
+// LDS floats of one wave: the tile (2d rows of kMaxChunk), the psi terms
+// and, for the symmetric pass 2, the tile frames' eps_x and eps_y.
+constexpr int lds_floats_per_wave(int d, int qpl, bool sym) {
+    return 2 * d * kMaxChunk + kWave * qpl + (sym ? 2 * kMaxChunk : 0);
 }
 
+// Dynamic LDS bytes a launch needs.
+inline size_t lds_bytes(int d, int qpl, int wpb, bool sym) {
+    return (size_t)wpb * (size_t)lds_floats_per_wave(d, qpl, sym) * sizeof(float);
+}
+
+// Scratch records (KeyIdx) per wave: the tie row, plus eps and counts for
+// the symmetric pass 2.
+inline long long scratch_records(long long nmax, bool sym) { return sym ? 3 * nmax : nmax; }
+
 // Kernel instances, one table per coordinate dimension (ksg_d{1,2,3}.hip).
-KernelFn kernel_d1(int k, int qpl);
-KernelFn kernel_d2(int k, int qpl);
-KernelFn kernel_d3(int k, int qpl);
+KernelFn kernel_d1(int k, int qpl, bool sym);
+KernelFn kernel_d2(int k, int qpl, bool sym);
+KernelFn kernel_d3(int k, int qpl, bool sym);
 
 #if defined(__HIPCC__)
 
@@ -267,13 +277,14 @@ __device__ __forceinline__ void pass1_tile_deferred(const Tile<D> &t, int cnt, i
     }
 }
 
-// Pass 2 over one staged tile: marginal counts within eps (self included).
+// Pass 2 over the tile frames [lo, hi): marginal counts within eps (self
+// included).
 template <int D, int QPL>
-__device__ __forceinline__ void pass2_tile(const Tile<D> &t, int cnt4,
+__device__ __forceinline__ void pass2_tile(const Tile<D> &t, int lo, int hi,
                                            const float (&qa)[QPL][D], const float (&qb)[QPL][D],
                                            const float (&ex)[QPL], const float (&ey)[QPL],
                                            int (&cx)[QPL], int (&cy)[QPL]) {
-    for (int jj = 0; jj < cnt4; jj += 4) {
+    for (int jj = lo; jj < hi; jj += 4) {
         float4 A[D], B[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -289,6 +300,62 @@ __device__ __forceinline__ void pass2_tile(const Tile<D> &t, int cnt4,
                 cx[q] += marg<D>(qa[q], xa) <= ex[q] ? 1 : 0;
                 cy[q] += marg<D>(qb[q], xb) <= ey[q] ? 1 : 0;
             }
+        }
+    }
+}
+
+// Symmetric pass 2 over the tile frames [lo, hi) (a multiple of 64 apart,
+// all after the wave's query block): for query i (lane) and frame j
+// (wave-uniform) one evaluation of dx = |x_i - x_j|_inf serves both
+//   n_x(i) += [dx <= eps_x(i)]  (lane counter), and
+//   n_x(j) += [dx <= eps_x(j)]  summed over the wave's valid queries by
+//                               ballot + popcount,
+// because |fl(a - b)| = |fl(b - a)|. Frame j's wave sum goes into lane
+// j % 64 of vx / vy (a lane-compare select) and every 64 frames one
+// coalesced atomic adds them to the item's count arrays Cx / Cy. NaN-padded frames (>= n)
+// fail every compare; their lanes are not flushed.
+template <int D, int QPL>
+__device__ __forceinline__ void pass2_sym(const Tile<D> &t, const float *tex, const float *tey,
+                                          int lo, int hi, int s, int n, int lane,
+                                          const bool (&valid)[QPL], const float (&qa)[QPL][D],
+                                          const float (&qb)[QPL][D], const float (&ex)[QPL],
+                                          const float (&ey)[QPL], int (&cx)[QPL],
+                                          int (&cy)[QPL], int *Cx, int *Cy) {
+    for (int g = lo; g < hi; g += kWave) {
+        int vx = 0, vy = 0;
+        for (int jj = g; jj < g + kWave; jj += 4) {
+            float4 A[D], B[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                A[d] = t.get(d, jj);
+                B[d] = t.get(D + d, jj);
+            }
+            const float4 EX = *reinterpret_cast<const float4 *>(tex + jj);
+            const float4 EY = *reinterpret_cast<const float4 *>(tey + jj);
+            const float exj[4] = {EX.x, EX.y, EX.z, EX.w};
+            const float eyj[4] = {EY.x, EY.y, EY.z, EY.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float xa[D], xb[D];
+                unpack<D>(A, B, u, xa, xb);
+                int nx = 0, ny = 0;
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) {
+                    const float da = marg<D>(qa[q], xa), db = marg<D>(qb[q], xb);
+                    cx[q] += da <= ex[q] ? 1 : 0;
+                    cy[q] += db <= ey[q] ? 1 : 0;
+                    nx += __popcll(__ballot(valid[q] && da <= exj[u]));
+                    ny += __popcll(__ballot(valid[q] && db <= eyj[u]));
+                }
+                const bool mine = lane == ((jj + u) & (kWave - 1));
+                vx = mine ? nx : vx;
+                vy = mine ? ny : vy;
+            }
+        }
+        const int f = s + g + lane;
+        if (f < n) {
+            if (vx) atomicAdd(Cx + f, vx);
+            if (vy) atomicAdd(Cy + f, vy);
         }
     }
 }
@@ -343,24 +410,52 @@ __device__ unsigned resolve_ties(bool tie, const float (&qa)[D], const float (&q
     return count;
 }
 
+// Stage the eps of tile frames [c0, c0 + chunk) (NaN beyond n) next to the
+// coordinates, for the symmetric pass 2.
+__device__ __forceinline__ void stage_eps(float *tex, float *tey, const float *Ex,
+                                          const float *Ey, int c0, int chunk, int n, int lane) {
+    const float qnan = __builtin_nanf("");
+    for (int h = 0; h < chunk; h += kWave) {
+        const int f = c0 + h + lane;
+        tex[h + lane] = f < n ? Ex[f] : qnan;
+        tey[h + lane] = f < n ? Ey[f] : qnan;
+    }
+}
+
+__device__ __forceinline__ void fence_agent() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Every wave walks items w = wave_gid, wave_gid + nwaves, ... All loop
 // bounds and item fields are wave-uniform scalars; the only lane-divergent
 // regions are the heap insertion, the tie rebuild's single-lane selection
 // and the per-lane loads/stores.
-template <int D, int K, int QPL>
+//
+// kSym = false: per query block, pass 1 then pass 2 over all frames, and the
+// block's psi terms are summed at once.
+// kSym = true: pass 1 and eps for every block first (eps to the item's
+// scratch, counts zeroed), then pass 2 per block over its own frames and
+// the frames after it only, with the symmetric count of pass2_sym, then the
+// psi terms of all frames in frame order. Per item the wave's scratch holds
+// [tie records: n | eps_x, eps_y: n floats each | n_x, n_y: n ints each].
+template <int D, int K, int QPL, bool kSym>
 __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
     extern __shared__ __attribute__((aligned(16))) float dyn_lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int chunk = a.chunk;
-    float *wave_lds = dyn_lds + wib * (2 * D * kMaxChunk + kWave * QPL);
+    float *wave_lds = dyn_lds + wib * lds_floats_per_wave(D, QPL, kSym);
     const Tile<D> tile{wave_lds, chunk};
     float *terms = wave_lds + 2 * D * kMaxChunk;
+    float *tex = terms + kWave * QPL;
+    float *tey = tex + kMaxChunk;
     const long long nwaves = (long long)gridDim.x * a.wpb;
     const long long wave_gid = (long long)blockIdx.x * a.wpb + wib;
     KeyIdx *scratch = a.scratch + wave_gid * a.scratch_stride;
     const long long total = (long long)a.R * a.P;
     const long long ds = a.dstride;
+    constexpr int kBlock = kWave * QPL;
     unsigned long long ties = 0;
 
     for (long long w = wave_gid; w < total; w += nwaves) {
@@ -370,9 +465,13 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
         const int n = a.win_n[slot];
         const float *xa = a.X + (long long)a.ab[2 * p] * a.ld + off;
         const float *xb = a.X + (long long)a.ab[2 * p + 1] * a.ld + off;
+        float *Ex = reinterpret_cast<float *>(scratch + a.scratch_n);
+        float *Ey = Ex + a.scratch_n;
+        int *Cx = reinterpret_cast<int *>(scratch + 2 * a.scratch_n);
+        int *Cy = Cx + a.scratch_n;
 
         float acc = 0.0f;
-        for (int q0 = 0; q0 < n; q0 += kWave * QPL) {
+        for (int q0 = 0; q0 < n; q0 += kBlock) {
             int i[QPL];
             bool valid[QPL];
             float qa[QPL][D], qb[QPL][D];
@@ -397,8 +496,7 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
             // Pass 1: k+1 nearest by joint distance.
             for (int c0 = 0; c0 < n; c0 += chunk) {
                 stage_tile<D>(tile, xa, xb, ds, c0, n, lane);
-                const int cnt4 = (min(chunk, n - c0) + 3) & ~3;
-                const bool self = (c0 < q0 + kWave * QPL) && (q0 < c0 + chunk);
+                const bool self = (c0 < q0 + kBlock) && (q0 < c0 + chunk);
                 const int cnt = min(chunk, n - c0);
                 if (self) pass1_tile_deferred<D, K, QPL, true>(tile, cnt, c0, i, qa, qb, h, id);
                 else pass1_tile_deferred<D, K, QPL, false>(tile, cnt, c0, i, qa, qb, h, id);
@@ -419,6 +517,17 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
                 ties += resolve_ties<D>(tie, qa[q], qb[q], xa, xb, ds, n, K, scratch, lane,
                                         ex[q], ey[q]);
             }
+            if (kSym) {
+#pragma unroll
+                for (int q = 0; q < QPL; ++q)
+                    if (valid[q]) {
+                        Ex[i[q]] = ex[q];
+                        Ey[i[q]] = ey[q];
+                        Cx[i[q]] = 0;
+                        Cy[i[q]] = 0;
+                    }
+                continue;
+            }
             // Pass 2: marginal counts.
             int cx[QPL], cy[QPL];
 #pragma unroll
@@ -426,7 +535,7 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
             for (int c0 = 0; c0 < n; c0 += chunk) {
                 stage_tile<D>(tile, xa, xb, ds, c0, n, lane);
                 const int cnt4 = (min(chunk, n - c0) + 3) & ~3;
-                pass2_tile<D, QPL>(tile, cnt4, qa, qb, ex, ey, cx, cy);
+                pass2_tile<D, QPL>(tile, 0, cnt4, qa, qb, ex, ey, cx, cy);
                 wave_sync();
             }
 #pragma unroll
@@ -441,9 +550,80 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
             wave_sync();
             // Sequential float32 sum in frame order (every lane runs the same
             // sum over LDS broadcasts).
-            const int lim = min(kWave * QPL, n - q0);
+            const int lim = min(kBlock, n - q0);
             for (int l = 0; l < lim; ++l) acc += terms[l];
             wave_sync();
+        }
+        if (kSym) {
+            fence_agent();  // eps and zeroed counts visible to every lane
+            for (int q0 = 0; q0 < n; q0 += kBlock) {
+                int i[QPL];
+                bool valid[QPL];
+                float qa[QPL][D], qb[QPL][D], ex[QPL], ey[QPL];
+                int cx[QPL], cy[QPL];
+#pragma unroll
+                for (int q = 0; q < QPL; ++q) {
+                    i[q] = q0 + q * kWave + lane;
+                    valid[q] = i[q] < n;
+                    const int iq = valid[q] ? i[q] : 0;
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        qa[q][d] = xa[d * ds + iq];
+                        qb[q][d] = xb[d * ds + iq];
+                    }
+                    ex[q] = Ex[iq];
+                    ey[q] = Ey[iq];
+                    cx[q] = cy[q] = 0;
+                }
+                // Frames of the own block count for the lane only; frames
+                // after it count both ways; frames before it were counted
+                // by earlier blocks.
+                const int own_end = min(q0 + kBlock, n);
+                for (int c0 = q0; c0 < n; c0 += chunk) {
+                    stage_tile<D>(tile, xa, xb, ds, c0, n, lane);
+                    const int real = min(chunk, n - c0);            // frames < n in the tile
+                    const int own = max(0, min(own_end - c0, chunk));  // own-block frames
+                    // Own frames: the lane's count only. Rounded up to 4 into
+                    // the NaN padding, which counts nothing (own < 4-multiple
+                    // only when the own block ends at n).
+                    if (own > 0)
+                        pass2_tile<D, QPL>(tile, 0, (own + 3) & ~3, qa, qb, ex, ey, cx, cy);
+                    // Frames after the own block: both ways. Here own_end < n,
+                    // so own_end, c0 and hence `own` are multiples of 64, and
+                    // the 64-frame groups end inside the tile.
+                    if (real > own) {
+                        stage_eps(tex, tey, Ex, Ey, c0, chunk, n, lane);
+                        wave_sync();
+                        pass2_sym<D, QPL>(tile, tex, tey, own, (real + kWave - 1) & ~(kWave - 1),
+                                          c0, n, lane, valid, qa, qb, ex, ey, cx, cy, Cx, Cy);
+                    }
+                    wave_sync();
+                }
+#pragma unroll
+                for (int q = 0; q < QPL; ++q)
+                    if (valid[q]) {
+                        atomicAdd(Cx + i[q], cx[q]);
+                        atomicAdd(Cy + i[q], cy[q]);
+                    }
+            }
+            fence_agent();  // every count complete
+            for (int f0 = 0; f0 < n; f0 += kWave) {
+                const int f = f0 + lane;
+                float term = 0.0f;
+                if (f < n) {
+                    const int nx = Cx[f] - 1, ny = Cy[f] - 1;
+                    term = a.psi[nx] + a.psi[ny];
+                    if (a.counts_out) {
+                        a.counts_out[(p * n + f) * 2 + 0] = nx;
+                        a.counts_out[(p * n + f) * 2 + 1] = ny;
+                    }
+                }
+                terms[lane] = term;
+                wave_sync();
+                const int lim = min(kWave, n - f0);
+                for (int l = 0; l < lim; ++l) acc += terms[l];
+                wave_sync();
+            }
         }
         const float avg = acc / (float)n;
         const float mi = a.psi[n] + a.psi[a.k] - a.inv_k - avg;
@@ -454,29 +634,31 @@ __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
 
 template <int D, int K>
 struct Instances {
-    static void fill(KernelFn (*t)[3]) {
-        t[K][1] = ksg_kernel<D, K, 1>;
-        t[K][2] = ksg_kernel<D, K, 2>;
+    static void fill(KernelFn (*t)[3][2]) {
+        t[K][1][0] = ksg_kernel<D, K, 1, false>;
+        t[K][2][0] = ksg_kernel<D, K, 2, false>;
+        t[K][1][1] = ksg_kernel<D, K, 1, true>;
+        t[K][2][1] = ksg_kernel<D, K, 2, true>;
         Instances<D, K - 1>::fill(t);
     }
 };
 template <int D>
 struct Instances<D, 0> {
-    static void fill(KernelFn (*)[3]) {}
+    static void fill(KernelFn (*)[3][2]) {}
 };
 
 // Table lookup shared by the per-dimension translation units.
 template <int D>
 struct InstanceTable {
-    KernelFn t[kMaxK + 1][3] = {};
+    KernelFn t[kMaxK + 1][3][2] = {};
     InstanceTable() { Instances<D, kMaxK>::fill(t); }
 };
 
 template <int D>
-KernelFn kernel_lookup(int k, int qpl) {
+KernelFn kernel_lookup(int k, int qpl, bool sym) {
     static const InstanceTable<D> tab;  // thread-safe static initialisation
     if (k < 1 || k > kMaxK || qpl < 1 || qpl > 2) return nullptr;
-    return tab.t[k][qpl];
+    return tab.t[k][qpl][sym ? 1 : 0];
 }
 
 #endif  // __HIPCC__

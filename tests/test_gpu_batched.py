@@ -8,7 +8,7 @@ from netsci_amd import abi, synth
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(q, c, w) for q in (1, 2) for c in (64, 128, 256) for w in (1, 4, 8)]
+SHAPES = [(q, c, w, s) for q in (1, 2) for c in (64, 128, 256) for w in (1, 4, 8) for s in (0, 1)]
 
 
 def _problem():
@@ -25,11 +25,11 @@ def test_every_launch_shape_is_bitexact():
     dab = abi.DeviceArray.from_numpy(ab)
     dmi = abi.DeviceArray((len(ab),), np.float32)
     try:
-        for q, c, w in SHAPES:
-            print(f"shape qpl={q} tile={c} wpb={w}", flush=True)
-            abi.set_config(q, c, w, 0)
+        for q, c, w, s in SHAPES:
+            print(f"shape qpl={q} tile={c} wpb={w} symmetric={s}", flush=True)
+            abi.set_config(q, c, w, 0, s)
             abi.ksg_mi(dX, X.shape[1], n, d, n, dab, len(ab), k, dmi)
-            np.testing.assert_array_equal(dmi.numpy(), want, err_msg=f"shape {(q, c, w)}")
+            np.testing.assert_array_equal(dmi.numpy(), want, err_msg=f"shape {(q, c, w, s)}")
     finally:
         abi.set_config(0)
 

@@ -14,6 +14,16 @@ from tests.conftest import curve_inputs
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=[0, 1], ids=["pass2", "sympass2"])
+def pass2_variant(request):
+    """Every test runs with the plain pass 2 (each ordered frame pair) and
+    with the symmetric one (each unordered pair once, both frames credited);
+    both must give the oracle's bits."""
+    abi.set_config(2, 128, 4, 0, request.param)
+    yield request.param
+    abi.set_config(0)
+
+
 def _flush(msg):
     print(msg, flush=True)
 
