@@ -207,6 +207,20 @@ int nsx_cpu_hedetniemi(const float *h_A, int N, int L, float tol, float *h_H,
                        int *h_paths, int threads);
 int nsx_cpu_corr_to_adj(const float *h_C, int N, float *h_A);
 
+/* ---- row statistics (netcalc/include/math_utils.h:24-101) ---------------
+ * meanGpu / standardDeviationGpu (math_utils_gpu.cu:158-206, kernels
+ * :23-156): per row of an m x n float32 matrix, the mean u and the
+ * population standard deviation sigma, summed in the reference's order
+ * (1024 strided partial sums folded as a tree), so the values are the
+ * reference kernel's bits (the square is fl(d * d), where the reference
+ * calls powf(d, 2)). n >= 1. */
+int nsx_row_mean(const float *d_a, long long m, long long n, float *d_u, void *stream);
+int nsx_row_std(const float *d_a, long long m, long long n, float *d_u, float *d_sigma,
+                void *stream);
+/* CPU platform: the same sums on the host (h_sigma nullable). The
+ * reference's CPU branch computes nothing (math_utils.cpp:6-35). */
+int nsx_cpu_row_stats(const float *h_a, long long m, long long n, float *h_u, float *h_sigma);
+
 #ifdef __cplusplus
 }
 #endif

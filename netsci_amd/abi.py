@@ -46,6 +46,9 @@ SIGNATURES = {
     "nsx_ksg_plan_config": [_VP, _IP, _IP, _IP, _LLP, ctypes.POINTER(ctypes.c_double), _IP],
     "nsx_ksg_plan_destroy": [_VP],
     "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
+    "nsx_row_mean": [_VP, _LL, _LL, _VP, _VP],
+    "nsx_row_std": [_VP, _LL, _LL, _VP, _VP, _VP],
+    "nsx_cpu_row_stats": [_VP, _LL, _LL, _VP, _VP],
     "nsx_psi_host": [_VP, _I],
     "nsx_screen_features": [_VP, _LL, _LL, _I, _I, _VP, _LL, _VP, _VP],
     "nsx_mlp_param_count": [_IP],

@@ -340,6 +340,69 @@ void correlationToAdjacency(CuArray<float> *A, CuArray<float> *C, int n, int pla
     }
 }
 
+// ---- row statistics (math_utils.cpp:6-35, math_utils_gpu.cu:158-206) ----
+
+static void check_rows(CuArray<float> *a, int m, int n) {
+    if (n < 1) throw std::invalid_argument("n must be >= 1");
+    if (m < 0 || (long long)m * n > (long long)a->size())
+        throw std::invalid_argument("a holds fewer than m x n values");
+}
+
+void meanGpu(CuArray<float> *a, CuArray<float> *u, int m, int n) {
+    check_rows(a, m, n);
+    if ((int)u->size() < m) u->init(1, m);
+    if (m == 0) return;
+    const size_t in = (size_t)m * n * sizeof(float), out = (size_t)m * sizeof(float);
+    DeviceBuffer da(in), du(out);
+    check(nsx_memcpy_h2d(da.ptr, a->host(), in), "upload a");
+    check(nsx_row_mean(da.as<float>(), m, n, du.as<float>(), nullptr), "row mean on the GPU");
+    check(nsx_memcpy_d2h(u->host(), du.ptr, out), "download u");
+}
+
+void standardDeviationGpu(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sigma, int m,
+                          int n) {
+    check_rows(a, m, n);
+    if ((int)u->size() < m) u->init(1, m);
+    if ((int)sigma->size() < m) sigma->init(1, m);
+    if (m == 0) return;
+    const size_t in = (size_t)m * n * sizeof(float), out = (size_t)m * sizeof(float);
+    DeviceBuffer da(in), du(out), ds(out);
+    check(nsx_memcpy_h2d(da.ptr, a->host(), in), "upload a");
+    check(nsx_row_std(da.as<float>(), m, n, du.as<float>(), ds.as<float>(), nullptr),
+          "row standard deviation on the GPU");
+    check(nsx_memcpy_d2h(u->host(), du.ptr, out), "download u");
+    check(nsx_memcpy_d2h(sigma->host(), ds.ptr, out), "download sigma");
+}
+
+void mean(CuArray<float> *a, CuArray<float> *u, int m, int n, int platform) {
+    if (n < 1) throw std::invalid_argument("n must be >= 1");
+    m = (int)(a->size() / n);
+    u->init(1, m);
+    if (platform == GPU_PLATFORM) {
+        meanGpu(a, u, m, n);
+    } else if (platform == CPU_PLATFORM) {
+        check(nsx_cpu_row_stats(a->host(), m, n, u->host(), nullptr), "row mean on the CPU");
+    } else {
+        throw std::invalid_argument("Invalid platform");
+    }
+}
+
+void standardDeviation(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sigma, int m, int n,
+                       int platform) {
+    if (n < 1) throw std::invalid_argument("n must be >= 1");
+    m = (int)(a->size() / n);
+    u->init(1, m);
+    sigma->init(1, m);
+    if (platform == GPU_PLATFORM) {
+        standardDeviationGpu(a, u, sigma, m, n);
+    } else if (platform == CPU_PLATFORM) {
+        check(nsx_cpu_row_stats(a->host(), m, n, u->host(), sigma->host()),
+              "row standard deviation on the CPU");
+    } else {
+        throw std::invalid_argument("Invalid platform");
+    }
+}
+
 std::vector<int> recoverSingleShortestPath(CuArray<int> *paths, int maxPathLength, int i, int j) {
     if (maxPathLength < 1) throw std::invalid_argument("maxPathLength must be >= 1");
     const int n = (int)std::sqrt((double)(paths->n() / maxPathLength));

@@ -57,6 +57,14 @@ void hedetniemiAllShortestPathLengthsGpu(CuArray<float> *A, CuArray<float> *H,
                                          int maxPathLength);
 void correlationToAdjacency(CuArray<float> *A, CuArray<float> *C, int n, int platform);
 void correlationToAdjacencyGpu(CuArray<float> *A, CuArray<float> *C, int n);
+// math_utils.h:24-101: per-row mean and population standard deviation of an
+// m x n matrix (m is recomputed as a.size() / n, as math_utils.cpp:14,32).
+void mean(CuArray<float> *a, CuArray<float> *u, int m, int n, int platform);
+void meanGpu(CuArray<float> *a, CuArray<float> *u, int m, int n);
+void standardDeviation(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sigma, int m, int n,
+                       int platform);
+void standardDeviationGpu(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sigma, int m,
+                          int n);
 // hedetniemi.cpp:71-108: the path i -> j as node indices.
 std::vector<int> recoverSingleShortestPath(CuArray<int> *paths, int maxPathLength, int i, int j);
 

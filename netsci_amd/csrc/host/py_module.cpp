@@ -193,6 +193,11 @@ PYBIND11_MODULE(_netsci, m) {
           "H"_a, "maxPathLength"_a);
     m.def("correlationToAdjacency", &correlationToAdjacency, "A"_a, "C"_a, "n"_a, "platform"_a);
     m.def("correlationToAdjacencyGpu", &correlationToAdjacencyGpu, "A"_a, "C"_a, "n"_a);
+    m.def("mean", &mean, "a"_a, "u"_a, "m"_a, "n"_a, "platform"_a);
+    m.def("meanGpu", &meanGpu, "a"_a, "u"_a, "m"_a, "n"_a);
+    m.def("standardDeviation", &standardDeviation, "a"_a, "u"_a, "sigma"_a, "m"_a, "n"_a,
+          "platform"_a);
+    m.def("standardDeviationGpu", &standardDeviationGpu, "a"_a, "u"_a, "sigma"_a, "m"_a, "n"_a);
     m.def(
         "recoverSingleShortestPath",
         [](CuArray<int> *paths, int maxPathLength, int i, int j) {

@@ -142,6 +142,44 @@ int nsx_gc_from_mi_host(const float *h_mi, long long P, int d, float *h_gc) {
 }
 
 // hedetniemi_gpu.cu:8-26 on the host.
+// Row mean / standard deviation in the summation order of
+// math_utils_gpu.cu:23-156 (1024 strided partial sums folded as a tree), so
+// the CPU platform gives the GPU platform's bits. The reference's CPU
+// branch computes nothing (math_utils.cpp:6-35).
+static float row_fold(const float *row, long long n, float shift, bool square) {
+    float s[1024];
+    for (int t = 0; t < 1024; ++t) {
+        float acc = 0.0f;
+        for (long long i = t; i < n; i += 1024) {
+            if (square) {
+                const float d = row[i] - shift;
+                acc += d * d;
+            } else {
+                acc += row[i];
+            }
+        }
+        s[t] = acc;
+    }
+    for (int o = 512; o >= 1; o >>= 1)
+        for (int t = 0; t < o; ++t) s[t] = s[t] + s[t + o];
+    return s[0];
+}
+
+int nsx_cpu_row_stats(const float *h_a, long long m, long long n, float *h_u, float *h_sigma) {
+    nsx::clear_error();
+    if (!h_a || !h_u || m < 0 || n < 1) {
+        nsx::set_error("nsx_cpu_row_stats: invalid arguments");
+        return 1;
+    }
+    parallel_for(m, 0, [&](long long r) {
+        const float *row = h_a + r * n;
+        const float mean = row_fold(row, n, 0.0f, false) / (float)n;
+        h_u[r] = mean;
+        if (h_sigma) h_sigma[r] = std::sqrt(row_fold(row, n, mean, true) / (float)n);
+    });
+    return 0;
+}
+
 int nsx_cpu_corr_to_adj(const float *h_C, int N, float *h_A) {
     nsx::clear_error();
     if (!h_C || !h_A || N < 0) {

@@ -27,11 +27,15 @@ from .._netsci import (  # noqa: F401
     hedetniemiAllShortestPaths,
     hedetniemiAllShortestPathsCpu,
     hedetniemiAllShortestPathsGpu,
+    mean,
+    meanGpu,
     mutualInformation,
     mutualInformationCpu,
     mutualInformationGpu,
     mutualInformationWithCheckpointing,
     recoverSingleShortestPath,
+    standardDeviation,
+    standardDeviationGpu,
     windowedCorrelation,
 )
 

@@ -134,6 +134,44 @@ void orc_corr_to_adj(const float *C, int N, float *A) {
         }
 }
 
+// math_utils_gpu.cu:23-68 (meanKernel) and :70-156 (standardDeviationKernel):
+// per row, thread t of 1024 sums a[t], a[t + 1024], ... in order; the 1024
+// partial sums are folded as a tree, s[t] += s[t + o] for o = 512, ..., 1
+// (the warp-synchronous tail of the reference is the same tree for s[0]);
+// u = s[0] / (float)n. The deviation pass sums (a - u)^2 in the same order
+// and sigma = sqrt(s[0] / (float)n). The reference squares with powf(x, 2);
+// this restatement squares as fl(x * x) (parity with the original binary
+// unpinned for sigma; every other step is exact float arithmetic).
+static float row_tree_sum(const float *row, long long n, float shift, bool square) {
+    float s[1024];
+    for (int t = 0; t < 1024; ++t) {
+        float acc = 0.0f;
+        for (long long i = t; i < n; i += 1024) {
+            if (square) {
+                const float d = row[i] - shift;
+                acc += d * d;
+            } else {
+                acc += row[i];
+            }
+        }
+        s[t] = acc;
+    }
+    for (int o = 512; o >= 1; o >>= 1)
+        for (int t = 0; t < o; ++t) s[t] = s[t] + s[t + o];
+    return s[0];
+}
+
+void orc_row_mean(const float *a, long long m, long long n, float *u) {
+    for (long long r = 0; r < m; ++r) u[r] = row_tree_sum(a + r * n, n, 0.0f, false) / (float)n;
+}
+
+void orc_row_std(const float *a, long long m, long long n, float *u, float *sigma) {
+    for (long long r = 0; r < m; ++r) {
+        u[r] = row_tree_sum(a + r * n, n, 0.0f, false) / (float)n;
+        sigma[r] = std::sqrt(row_tree_sum(a + r * n, n, u[r], true) / (float)n);
+    }
+}
+
 // hedetniemi_gpu.cu:47-68 (Part-1) in Jacobi form, and :70-107 (Part-2).
 // H_{p+1}[i][j] = min_k fl(A[j][k] + H_p[k][i]) taken with a strict `<`
 // from +inf in ascending k; Hi[p] = H_p (H_0 = A). Part-2 backtracks each

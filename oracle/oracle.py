@@ -41,6 +41,8 @@ def lib():
                                 ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, _f32p]
         L.orc_corr_to_adj.argtypes = [_f32p, ctypes.c_int, _f32p]
+        L.orc_row_mean.argtypes = [_f32p, ctypes.c_longlong, ctypes.c_longlong, _f32p]
+        L.orc_row_std.argtypes = [_f32p, ctypes.c_longlong, ctypes.c_longlong, _f32p, _f32p]
         L.orc_hedetniemi.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                      _f32p, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_recover_path.argtypes = [_i32p, ctypes.c_int, ctypes.c_int,
@@ -106,6 +108,23 @@ def corr_to_adj(C):
     A = np.zeros_like(C)
     lib().orc_corr_to_adj(C, C.shape[0], A)
     return A
+
+
+def row_mean(a):
+    """math_utils_gpu.cu:23-68: per-row mean in the reference's summation order."""
+    a = _f32(a)
+    u = np.zeros(a.shape[0], np.float32)
+    lib().orc_row_mean(a, a.shape[0], a.shape[1], u)
+    return u
+
+
+def row_std(a):
+    """math_utils_gpu.cu:70-156: per-row mean and population deviation."""
+    a = _f32(a)
+    u = np.zeros(a.shape[0], np.float32)
+    sigma = np.zeros(a.shape[0], np.float32)
+    lib().orc_row_std(a, a.shape[0], a.shape[1], u, sigma)
+    return u, sigma
 
 
 def hedetniemi(A, L, tol, want_hi=False, want_paths=True):
