@@ -170,6 +170,33 @@ def test_full_size_symmetry_and_sample():
         assert mi[p] == np.float32(oracle.mi(X[a], X[b], k, n, d))
 
 
+_LONG = {}
+
+
+def _long_window_reference():
+    """Oracle counts for two pairs at n = 9001 (computed once per session)."""
+    if not _LONG:
+        N, n, d, k = 3, 9001, 3, 6
+        X, _ = synth.calpha_trajectory(N=N, n=n, seed=77)
+        ab = np.array([[0, 1], [1, 2]], np.int32)
+        ref = [oracle.ksg_counts(X[a], X[b], k, n, d)[:2] for a, b in ab]
+        mi = [np.float32(oracle.mi_from_counts(na, nb, k, n)) for na, nb in ref]
+        _LONG.update(X=X, ab=ab, n=n, d=d, k=k, ref=ref, mi=mi)
+    return _LONG
+
+
+def test_long_window_counts_bitexact():
+    """n = 9001 frames (35 LDS tiles of 256 plus a ragged tail, 71 query
+    blocks of 128): per-frame counts and MI equal the oracle's."""
+    _flush("long window n=9001")
+    L = _long_window_reference()
+    mi, cnt, _ = _gpu_counts(L["X"], L["ab"], L["k"], L["n"], L["d"])
+    for p in range(2):
+        np.testing.assert_array_equal(cnt[p, :, 0], L["ref"][p][0])
+        np.testing.assert_array_equal(cnt[p, :, 1], L["ref"][p][1])
+        assert mi[p] == L["mi"][p]
+
+
 def test_self_pair_closed_form():
     """Xa == Xb without duplicate frames: n_x = n_y = k for every frame, so
     MI = psi[n] + psi[k] - 1/k - (sum of 2 psi[k] in frame order)/n."""
