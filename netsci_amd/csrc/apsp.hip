@@ -335,25 +335,30 @@ struct __attribute__((aligned(8))) Span {
 // Sparse backtrack: one lane per (i, j); a workgroup of 1024 lanes takes
 // one i and 1024 consecutive j. Every step p all its lanes read from the row
 // Hi[p][i][.], so the workgroup stages that row in LDS (kLdsRow; N <=
-// kLdsRowMax) and the per-candidate reads become LDS reads. Workgroups
-// g, g+8, ... share an XCD and get the same i, so the row comes from that
-// XCD's L2 for all but the first of them. Writes paths[i][j][0..L-1]
-// (diagonal: i).
+// kLdsRowMax) and the per-candidate reads become LDS reads. Two row buffers:
+// while the lanes scan step p from one, each lane's share of row p-1 is
+// already loading into registers and goes to the other buffer afterwards,
+// so one barrier per step separates the scans and the row fetch is hidden
+// behind them. Workgroups g, g+8, ... share an XCD and get the same i, so the
+// row comes from that XCD's L2 for all but the first of them. Writes
+// paths[i][j][0..L-1] (diagonal: i).
 constexpr int kBtThreads = 1024;
-constexpr int kLdsRowMax = 16384;
+constexpr int kLdsRowMax = 8192;
+constexpr int kRowPerLane = kLdsRowMax / kBtThreads;
 
 template <bool kLdsRow>
 __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
     const float *__restrict__ H, const float *__restrict__ Hi, const long long *__restrict__ rows,
     const int *__restrict__ lst, const unsigned *__restrict__ dense, int *__restrict__ paths,
     float tol, int N, int L, int nC) {
-    extern __shared__ float hrow_lds[];
+    extern __shared__ float hrow_lds[];  // kLdsRow: 2 x N floats
     if (dense[0]) return;
     const int g = blockIdx.x;
     const int s = g >> 3;
     const int i = (g & 7) + 8 * (s / nC);
     if (i >= N) return;
-    const int j = (s % nC) * kBtThreads + threadIdx.x;
+    const int t0 = threadIdx.x;
+    const int j = (s % nC) * kBtThreads + t0;
     const long long NN = (long long)N * N;
     const bool active = j < N && j != i;
     int *pp = paths + ((long long)i * N + (j < N ? j : 0)) * L;
@@ -361,34 +366,54 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
         for (int p = 0; p < L; ++p) pp[p] = i;
     float len = active ? H[(long long)i * N + j] : 0.0f;
     int k = j;
+    float next[kRowPerLane];
+    if (kLdsRow) {
+        const float *row = Hi + (long long)(L - 1) * NN + (long long)i * N;
+        for (int t = t0; t < N; t += kBtThreads) hrow_lds[t] = row[t];
+        __syncthreads();
+    }
     for (int p = L - 1; p >= 0; --p) {
         const float *hrow = Hi + (long long)p * NN + (long long)i * N;
-        if (kLdsRow) {
-            __syncthreads();  // the previous step's reads are done
-            for (int t = threadIdx.x; t < N; t += kBtThreads) hrow_lds[t] = hrow[t];
-            __syncthreads();
+        const float *cur = hrow_lds + ((L - 1 - p) & 1) * N;
+        if (kLdsRow && p > 0) {
+            const float *row = hrow - NN;  // Hi[p-1][i][.]
+#pragma unroll
+            for (int r = 0; r < kRowPerLane; ++r) {
+                const int t = t0 + r * kBtThreads;
+                next[r] = t < N ? row[t] : 0.0f;
+            }
         }
-        if (!active) continue;
-        const Span sp = *reinterpret_cast<const Span *>(rows + k);
-        int found = -1;
-        float af = 0.0f;
-        for (long long e = sp.beg; e < sp.end; e += 4) {
-            const int4 q = *reinterpret_cast<const int4 *>(lst + 2 * e);
-            const float4 a = *reinterpret_cast<const float4 *>(lst + 2 * e + 4);
-            const float h0 = kLdsRow ? hrow_lds[q.x] : hrow[q.x];
-            const float h1 = kLdsRow ? hrow_lds[q.y] : hrow[q.y];
-            const float h2 = kLdsRow ? hrow_lds[q.z] : hrow[q.z];
-            const float h3 = kLdsRow ? hrow_lds[q.w] : hrow[q.w];
-            if (q.x != j && (h0 + a.x) - len < tol) { found = q.x; af = a.x; break; }
-            if (q.y != j && (h1 + a.y) - len < tol) { found = q.y; af = a.y; break; }
-            if (q.z != j && (h2 + a.z) - len < tol) { found = q.z; af = a.z; break; }
-            if (q.w != j && (h3 + a.w) - len < tol) { found = q.w; af = a.w; break; }
+        if (active) {
+            const Span sp = *reinterpret_cast<const Span *>(rows + k);
+            int found = -1;
+            float af = 0.0f;
+            for (long long e = sp.beg; e < sp.end; e += 4) {
+                const int4 q = *reinterpret_cast<const int4 *>(lst + 2 * e);
+                const float4 a = *reinterpret_cast<const float4 *>(lst + 2 * e + 4);
+                const float h0 = kLdsRow ? cur[q.x] : hrow[q.x];
+                const float h1 = kLdsRow ? cur[q.y] : hrow[q.y];
+                const float h2 = kLdsRow ? cur[q.z] : hrow[q.z];
+                const float h3 = kLdsRow ? cur[q.w] : hrow[q.w];
+                if (q.x != j && (h0 + a.x) - len < tol) { found = q.x; af = a.x; break; }
+                if (q.y != j && (h1 + a.y) - len < tol) { found = q.y; af = a.y; break; }
+                if (q.z != j && (h2 + a.z) - len < tol) { found = q.z; af = a.z; break; }
+                if (q.w != j && (h3 + a.w) - len < tol) { found = q.w; af = a.w; break; }
+            }
+            if (found >= 0) {
+                len -= af;
+                k = found;
+            }
+            pp[p] = k;
         }
-        if (found >= 0) {
-            len -= af;
-            k = found;
+        if (kLdsRow && p > 0) {
+            float *nxt = hrow_lds + ((L - p) & 1) * N;
+#pragma unroll
+            for (int r = 0; r < kRowPerLane; ++r) {
+                const int t = t0 + r * kBtThreads;
+                if (t < N) nxt[t] = next[r];
+            }
+            __syncthreads();  // scans of `cur` done, `nxt` complete
         }
-        pp[p] = k;
     }
 }
 
@@ -586,8 +611,8 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     const int bt_groups = 8 * ((N + 7) / 8) * nC;
     if (N <= kLdsRowMax)
         hipLaunchKernelGGL(backtrack_sparse_kernel<true>, dim3(bt_groups), dim3(kBtThreads),
-                           (size_t)N * sizeof(float), s, d_H, d_Hi, lt.rows, lt.lst, lt.dense,
-                           d_paths, tol, N, L, nC);
+                           2 * (size_t)N * sizeof(float), s, d_H, d_Hi, lt.rows, lt.lst,
+                           lt.dense, d_paths, tol, N, L, nC);
     else
         hipLaunchKernelGGL(backtrack_sparse_kernel<false>, dim3(bt_groups), dim3(kBtThreads), 0,
                            s, d_H, d_Hi, lt.rows, lt.lst, lt.dense, d_paths, tol, N, L, nC);
