@@ -76,7 +76,10 @@ int nsx_event_elapsed_ms(void *start, void *stop, float *ms);
  * equal bit-for-bit to mutualInformationCpu
  * (netcalc/src/mutual_information_cpu.cpp:6-121), ties at the k-th
  * neighbour included. 1 <= d <= 3, 1 <= k <= 16, k < n.
- * d_counts_out (nullable) receives n_x, n_y per frame: [p][f][2] int32. */
+ * d_counts_out (nullable) receives n_x, n_y per frame: [p][f][2] int32.
+ * Coordinates are expected finite: with a NaN coordinate the marginal max
+ * here drops the NaN, where the reference's std::max keeps it when it sits
+ * in the first dimension. */
 int nsx_ksg_mi(const float *d_X, long long ld, long long dstride, int d,
                int n, const int *d_ab, long long P, int k, float *d_mi_out,
                int *d_counts_out, void *stream);
