@@ -76,7 +76,8 @@ def build(verbose=False):
     objs = [o for _, o in jobs]
     tmp = lib_path() + ".tmp"
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs +
-         ["-pthread", "-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-pthread", "-L/opt/rocm/lib", "-lrocblas", "-lrocprofiler-sdk-roctx",
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, lib_path())
 
     import pybind11

@@ -550,6 +550,7 @@ using namespace nsx;
 extern "C" {
 
 int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *stream) {
+    TraceRange trace("nsx_hedetniemi_lengths");
     clear_error();
     NSX_REQUIRE(d_A && d_H && N >= 1 && L >= 0, "nsx_hedetniemi_lengths: invalid arguments");
     int rc = ensure_device();
@@ -576,6 +577,7 @@ int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *str
 
 int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float *d_Hi,
                    int *d_paths, void *stream) {
+    TraceRange trace("nsx_hedetniemi");
     clear_error();
     NSX_REQUIRE(d_A && d_H && N >= 1 && L >= 1, "nsx_hedetniemi: invalid arguments");
     NSX_REQUIRE(!d_paths || d_Hi, "nsx_hedetniemi: paths need the Hi snapshot buffer");
@@ -624,6 +626,7 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
 }
 
 int nsx_corr_to_adj(const float *d_C, int N, float *d_A, void *stream) {
+    TraceRange trace("nsx_corr_to_adj");
     clear_error();
     NSX_REQUIRE(d_C && d_A && N >= 0, "nsx_corr_to_adj: invalid arguments");
     if (N == 0) return kOk;

@@ -352,6 +352,7 @@ extern "C" {
 int nsx_ksg_mi(const float *d_X, long long ld, long long dstride, int d, int n,
                const int *d_ab, long long P, int k, float *d_mi_out, int *d_counts_out,
                void *stream) {
+    TraceRange trace("nsx_ksg_mi");
     clear_error();
     const int off = 0;
     Problem pb;
@@ -363,6 +364,7 @@ int nsx_ksg_mi(const float *d_X, long long ld, long long dstride, int d, int n,
 int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
                        const int *h_win_off, const int *h_win_n, int R, const int *d_ab,
                        long long P, int k, float *d_mi_out, void *stream) {
+    TraceRange trace("nsx_ksg_mi_windows");
     clear_error();
     NSX_REQUIRE(h_win_off && h_win_n && R >= 1, "ksg windows: bad window arrays");
     Problem pb;
@@ -395,6 +397,7 @@ int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symme
 int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long long dstride,
                         int d, const int *h_win_off, const int *h_win_n, int R,
                         const int *d_ab, long long P, int k, float *d_mi_out, int autotune) {
+    TraceRange trace("nsx_ksg_plan_create");
     clear_error();
     NSX_REQUIRE(plan && d_mi_out && h_win_off && h_win_n && R >= 1 && P >= 1,
                 "nsx_ksg_plan_create: invalid arguments");
@@ -492,6 +495,7 @@ int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long
 }
 
 int nsx_ksg_plan_launch(nsx_ksg_plan plan, void *stream) {
+    TraceRange trace("nsx_ksg_plan_launch");
     clear_error();
     NSX_REQUIRE(plan && plan->exec, "nsx_ksg_plan_launch: no plan");
     NSX_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream), "hipGraphLaunch");

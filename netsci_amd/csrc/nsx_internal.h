@@ -6,8 +6,19 @@
 #include <string>
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 namespace nsx {
+
+// A named roctx range for the lifetime of the object: each C-ABI entry
+// point opens one, so `rocprofv3 --marker-trace` attributes device time to
+// library calls. Without an attached profiler the calls return at once.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange &) = delete;
+    TraceRange &operator=(const TraceRange &) = delete;
+};
 
 // Thread-local last-error message behind nsx_last_error().
 void set_error(const std::string &msg);

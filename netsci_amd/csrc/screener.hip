@@ -513,6 +513,7 @@ int nsx_mlp_param_count(int *count) {
 
 int nsx_screen_features(const float *d_X, long long ld, long long dstride, int N, int n,
                         const int *d_ab, long long P, unsigned short *d_feat, void *stream) {
+    TraceRange trace("nsx_screen_features");
     clear_error();
     NSX_REQUIRE(d_X && d_ab && d_feat && N >= 1 && n >= 2 && P >= 0,
                 "nsx_screen_features: invalid arguments");
@@ -558,6 +559,7 @@ int nsx_screen_features(const float *d_X, long long ld, long long dstride, int N
 
 int nsx_mlp_forward(const float *d_params, const unsigned short *d_feat, long long P,
                     float *d_out, void *stream) {
+    TraceRange trace("nsx_mlp_forward");
     clear_error();
     NSX_REQUIRE(d_params && d_feat && d_out && P >= 0, "nsx_mlp_forward: invalid arguments");
     if (P == 0) return kOk;
@@ -573,6 +575,7 @@ int nsx_mlp_forward(const float *d_params, const unsigned short *d_feat, long lo
 
 int nsx_mlp_grad(const float *d_params, const unsigned short *d_feat, const float *d_y,
                  const float *d_w, long long B, float *d_grad, float *d_loss, void *stream) {
+    TraceRange trace("nsx_mlp_grad");
     clear_error();
     NSX_REQUIRE(d_params && d_feat && d_y && d_w && d_grad && B >= 1,
                 "nsx_mlp_grad: invalid arguments");
@@ -596,6 +599,7 @@ int nsx_mlp_grad(const float *d_params, const unsigned short *d_feat, const floa
 
 int nsx_mlp_adam(float *d_params, const float *d_grad, float *d_m, float *d_v, int step,
                  float lr, void *stream) {
+    TraceRange trace("nsx_mlp_adam");
     clear_error();
     NSX_REQUIRE(d_params && d_grad && d_m && d_v && step >= 1, "nsx_mlp_adam: invalid arguments");
     const float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.999f, (float)step);
@@ -609,6 +613,7 @@ int nsx_screen_select(const float *d_pred, long long P, float threshold, float a
                       unsigned seed, const int *d_ab, int *d_ab_out, long long *d_idx_out,
                       unsigned char *d_flag, long long *h_n_selected, long long *h_n_audit,
                       void *stream) {
+    TraceRange trace("nsx_screen_select");
     clear_error();
     NSX_REQUIRE(d_pred && d_ab && d_ab_out && d_idx_out && d_flag && P >= 0 && audit_rate >= 0 &&
                     audit_rate <= 1,

@@ -83,12 +83,14 @@ using namespace nsx;
 extern "C" {
 
 int nsx_row_mean(const float *d_a, long long m, long long n, float *d_u, void *stream) {
+    TraceRange trace("nsx_row_mean");
     clear_error();
     return launch_row_stats(d_a, m, n, d_u, nullptr, stream);
 }
 
 int nsx_row_std(const float *d_a, long long m, long long n, float *d_u, float *d_sigma,
                 void *stream) {
+    TraceRange trace("nsx_row_std");
     clear_error();
     NSX_REQUIRE(d_sigma, "nsx_row_std: null sigma");
     return launch_row_stats(d_a, m, n, d_u, d_sigma, stream);
