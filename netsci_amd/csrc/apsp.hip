@@ -152,18 +152,19 @@ __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ 
 // where nbr(j) = { k : A[j][k] < +inf }: every other k gives +inf or NaN,
 // which the select never takes, so the result equals the dense fold bit for
 // bit (signed zeros and NaN included).
-// Workgroup = a 64 (i) x 64 (j) output tile; lanes own i, each wave takes 16
-// of the j. For one j the neighbour list is wave-uniform (scalar loads) and
-// every k reads the coalesced row segment Hp[k][i0..i0+63]. Results go
+// Workgroup = a 64 (i) x 64 (j) output tile of 16 waves; lanes own i, each
+// wave takes 4 of the j (16 waves rather than 4 keep enough row reads in
+// flight at small N: 0.77 -> 0.41 ms for 32 steps at N = 1000). For one j
+// the neighbour list is wave-uniform (scalar loads) and every k reads the
+// coalesced row segment Hp[k][i0..i0+63]. Results go
 // through an LDS tile so the output is written as row segments
 // out[i][j0..j0+63]. Workgroups g, g+8, g+16, ... share an XCD; they get the
 // same i-range, so that XCD's L2 keeps the 64-column slice of Hp they read.
-__global__ __launch_bounds__(256) void minplus_sparse_kernel(const float *__restrict__ Hp,
-                                                             const long long *__restrict__ rows,
-                                                             const int *__restrict__ lst,
-                                                             const unsigned *__restrict__ dense,
-                                                             float *__restrict__ out, int N,
-                                                             int nI, int nJ) {
+constexpr int kStepWaves = 16;  // waves per sparse-step workgroup
+
+__global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
+    const float *__restrict__ Hp, const long long *__restrict__ rows, const int *__restrict__ lst,
+    const unsigned *__restrict__ dense, float *__restrict__ out, int N, int nI, int nJ) {
     __shared__ float T[64][65];
     if (dense[0]) return;
     const int g = blockIdx.x;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(256) void minplus_sparse_kernel(const float *__rest
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = i0 + lane;
     const float *col = Hp + (i < N ? i : 0);  // col[k * N] = Hp[k][i]
-    for (int jj = w; jj < 64; jj += 4) {
+    for (int jj = w; jj < 64; jj += kStepWaves) {
         const int j = j0 + jj;
         float c = __builtin_inff();
         if (j < N) {
@@ -218,7 +219,7 @@ __global__ __launch_bounds__(256) void minplus_sparse_kernel(const float *__rest
         T[lane][jj] = c;
     }
     __syncthreads();
-    for (int r = w; r < 64; r += 4) {
+    for (int r = w; r < 64; r += kStepWaves) {
         const int ii = i0 + r, j = j0 + lane;
         if (ii < N && j < N) out[(long long)ii * N + j] = T[r][lane];
     }
@@ -342,16 +343,49 @@ struct __attribute__((aligned(8))) Span {
 // behind them. Workgroups g, g+8, ... share an XCD and get the same i, so the
 // row comes from that XCD's L2 for all but the first of them. Writes
 // paths[i][j][0..L-1] (diagonal: i).
+// Measured at N = 4000, L = 132 (7.5 A contact graph): one dword store per
+// step 58.7 ms; non-temporal dword stores 109 ms; 16-byte non-temporal
+// blocks 44.5 ms (plain 16-byte stores 47.1); list starts in LDS 44.0;
+// two list groups in flight per iteration 37.8 ms; four need 77 VGPRs,
+// which halves the resident waves: 54.8 ms.
 constexpr int kBtThreads = 1024;
 constexpr int kLdsRowMax = 8192;
 constexpr int kRowPerLane = kLdsRowMax / kBtThreads;
 
+typedef int nsx_v4i __attribute__((ext_vector_type(4)));
+
+// First entry of one list group {q0..q3 | a0..a3} that passes the test, in
+// slot order; -1 if none.
 template <bool kLdsRow>
+__device__ __forceinline__ int first_pass(const int4 q, const float4 a, const float *cur,
+                                          const float *hrow, int j, float len, float tol,
+                                          float &af) {
+    const float h0 = kLdsRow ? cur[q.x] : hrow[q.x];
+    const float h1 = kLdsRow ? cur[q.y] : hrow[q.y];
+    const float h2 = kLdsRow ? cur[q.z] : hrow[q.z];
+    const float h3 = kLdsRow ? cur[q.w] : hrow[q.w];
+    if (q.x != j && (h0 + a.x) - len < tol) { af = a.x; return q.x; }
+    if (q.y != j && (h1 + a.y) - len < tol) { af = a.y; return q.y; }
+    if (q.z != j && (h2 + a.z) - len < tol) { af = a.z; return q.z; }
+    if (q.w != j && (h3 + a.w) - len < tol) { af = a.w; return q.w; }
+    return -1;
+}
+
+// kLdsRow: Hi[p][i][.] staged in LDS (two row buffers, N <= kLdsRowMax).
+// kLdsSpan: the list starts (in groups of 4 entries) of every node follow
+// the row buffers in LDS, so a step's list lookup is an LDS read.
+// kQuad: steps whose paths entries fill an aligned 16-byte block inside the
+// lane's own paths[i][j][0..L) are collected in registers and written as one
+// 16-byte non-temporal store, the partial blocks at either end as dwords
+// (needs a 16-byte aligned paths); otherwise one dword store per step.
+// kGroups: list groups whose loads are issued together before their tests
+// (loads past the list's end are skipped); the tests keep the slot order.
+template <bool kLdsRow, bool kLdsSpan, bool kQuad, int kGroups>
 __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
     const float *__restrict__ H, const float *__restrict__ Hi, const long long *__restrict__ rows,
     const int *__restrict__ lst, const unsigned *__restrict__ dense, int *__restrict__ paths,
     float tol, int N, int L, int nC) {
-    extern __shared__ float hrow_lds[];  // kLdsRow: 2 x N floats
+    extern __shared__ float hrow_lds[];  // kLdsRow: 2 x N floats; kLdsSpan: then N + 1 ints
     if (dense[0]) return;
     const int g = blockIdx.x;
     const int s = g >> 3;
@@ -361,17 +395,22 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
     const int j = (s % nC) * kBtThreads + t0;
     const long long NN = (long long)N * N;
     const bool active = j < N && j != i;
-    int *pp = paths + ((long long)i * N + (j < N ? j : 0)) * L;
+    const long long base = ((long long)i * N + (j < N ? j : 0)) * L;
+    int *pp = paths + base;
     if (j < N && j == i)
         for (int p = 0; p < L; ++p) pp[p] = i;
     float len = active ? H[(long long)i * N + j] : 0.0f;
     int k = j;
     float next[kRowPerLane];
+    int *span_lds = reinterpret_cast<int *>(hrow_lds + 2 * N);
+    if (kLdsSpan)
+        for (int t = t0; t <= N; t += kBtThreads) span_lds[t] = (int)(rows[t] >> 2);
     if (kLdsRow) {
         const float *row = Hi + (long long)(L - 1) * NN + (long long)i * N;
         for (int t = t0; t < N; t += kBtThreads) hrow_lds[t] = row[t];
-        __syncthreads();
     }
+    if (kLdsRow || kLdsSpan) __syncthreads();
+    nsx_v4i quad = {0, 0, 0, 0};
     for (int p = L - 1; p >= 0; --p) {
         const float *hrow = Hi + (long long)p * NN + (long long)i * N;
         const float *cur = hrow_lds + ((L - 1 - p) & 1) * N;
@@ -384,26 +423,59 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
             }
         }
         if (active) {
-            const Span sp = *reinterpret_cast<const Span *>(rows + k);
+            long long gb, ge;  // groups [gb, ge) of k's list
+            if (kLdsSpan) {
+                gb = span_lds[k];
+                ge = span_lds[k + 1];
+            } else {
+                const Span sp = *reinterpret_cast<const Span *>(rows + k);
+                gb = sp.beg >> 2;
+                ge = sp.end >> 2;
+            }
             int found = -1;
             float af = 0.0f;
-            for (long long e = sp.beg; e < sp.end; e += 4) {
-                const int4 q = *reinterpret_cast<const int4 *>(lst + 2 * e);
-                const float4 a = *reinterpret_cast<const float4 *>(lst + 2 * e + 4);
-                const float h0 = kLdsRow ? cur[q.x] : hrow[q.x];
-                const float h1 = kLdsRow ? cur[q.y] : hrow[q.y];
-                const float h2 = kLdsRow ? cur[q.z] : hrow[q.z];
-                const float h3 = kLdsRow ? cur[q.w] : hrow[q.w];
-                if (q.x != j && (h0 + a.x) - len < tol) { found = q.x; af = a.x; break; }
-                if (q.y != j && (h1 + a.y) - len < tol) { found = q.y; af = a.y; break; }
-                if (q.z != j && (h2 + a.z) - len < tol) { found = q.z; af = a.z; break; }
-                if (q.w != j && (h3 + a.w) - len < tol) { found = q.w; af = a.w; break; }
+            for (long long gi = gb; gi < ge && found < 0; gi += kGroups) {
+                int4 q[kGroups];
+                float4 a[kGroups];
+#pragma unroll
+                for (int u = 0; u < kGroups; ++u) {
+                    if (u == 0 || gi + u < ge) {
+                        q[u] = *reinterpret_cast<const int4 *>(lst + 8 * (gi + u));
+                        a[u] = *reinterpret_cast<const float4 *>(lst + 8 * (gi + u) + 4);
+                    } else {
+                        q[u] = make_int4(0, 0, 0, 0);
+                        a[u] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(),
+                                           __builtin_inff());
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kGroups; ++u)
+                    if (found < 0) found = first_pass<kLdsRow>(q[u], a[u], cur, hrow, j, len, tol, af);
             }
             if (found >= 0) {
                 len -= af;
                 k = found;
             }
-            pp[p] = k;
+            if (kQuad) {
+                // a = index of paths[i][j][p] in the int array; its aligned
+                // block [a & ~3, (a & ~3) + 3] lies inside this lane's
+                // entries [base, base + L) or is written entry by entry.
+                const long long a = base + p;
+                const long long b0 = a & ~3LL;
+                const int slot = (int)(a & 3);
+                if (b0 >= base && b0 + 3 < base + L) {
+                    quad.x = slot == 0 ? k : quad.x;
+                    quad.y = slot == 1 ? k : quad.y;
+                    quad.z = slot == 2 ? k : quad.z;
+                    quad.w = slot == 3 ? k : quad.w;
+                    if (slot == 0)
+                        __builtin_nontemporal_store(quad, reinterpret_cast<nsx_v4i *>(paths + b0));
+                } else {
+                    pp[p] = k;
+                }
+            } else {
+                pp[p] = k;
+            }
         }
         if (kLdsRow && p > 0) {
             float *nxt = hrow_lds + ((L - p) & 1) * N;
@@ -511,8 +583,8 @@ int minplus_step(const float *A, const float *Hp, float *out, int N, const unsig
                  const Lists &la, hipStream_t s) {
     const int nT = (N + 63) / 64;
     const int groups = 8 * ((nT + 7) / 8) * nT;
-    hipLaunchKernelGGL(minplus_sparse_kernel, dim3(groups), dim3(256), 0, s, Hp, la.rows, la.lst,
-                       la.dense, out, N, nT, nT);
+    hipLaunchKernelGGL(minplus_sparse_kernel, dim3(groups), dim3(kStepWaves * 64), 0, s, Hp,
+                       la.rows, la.lst, la.dense, out, N, nT, nT);
     NSX_HIP(hipGetLastError(), "sparse min-plus launch");
     const int tiles = (N + kTile - 1) / kTile;
     hipLaunchKernelGGL(minplus_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, Hp, out, N,
@@ -520,6 +592,7 @@ int minplus_step(const float *A, const float *Hp, float *out, int N, const unsig
     NSX_HIP(hipGetLastError(), "min-plus launch");
     return kOk;
 }
+
 
 int negzero_flag(const float *A, int N, unsigned **flag, hipStream_t s) {
     unsigned *f = (unsigned *)workspace(kWsApsp1, 256);
@@ -611,13 +684,18 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     if (rc) return rc;
     const int nC = (N + kBtThreads - 1) / kBtThreads;
     const int bt_groups = 8 * ((N + 7) / 8) * nC;
-    if (N <= kLdsRowMax)
-        hipLaunchKernelGGL(backtrack_sparse_kernel<true>, dim3(bt_groups), dim3(kBtThreads),
-                           2 * (size_t)N * sizeof(float), s, d_H, d_Hi, lt.rows, lt.lst,
-                           lt.dense, d_paths, tol, N, L, nC);
-    else
-        hipLaunchKernelGGL(backtrack_sparse_kernel<false>, dim3(bt_groups), dim3(kBtThreads), 0,
-                           s, d_H, d_Hi, lt.rows, lt.lst, lt.dense, d_paths, tol, N, L, nC);
+    const size_t row_lds = 2 * (size_t)N * sizeof(float);
+    const size_t span_lds = row_lds + (size_t)(N + 1) * sizeof(int);
+    const bool quad = ((uintptr_t)d_paths & 15) == 0;
+#define NSX_BT(R, S, W, G, lds)                                                                  \
+    hipLaunchKernelGGL((backtrack_sparse_kernel<R, S, W, G>), dim3(bt_groups), dim3(kBtThreads), \
+                       lds, s, d_H, d_Hi, lt.rows, lt.lst, lt.dense, d_paths, tol, N, L, nC)
+    if (N > kLdsRowMax && quad) NSX_BT(false, false, true, 2, 0);
+    else if (N > kLdsRowMax) NSX_BT(false, false, false, 2, 0);
+    else if (!quad) NSX_BT(true, false, false, 2, row_lds);
+    else if (span_lds > 65536) NSX_BT(true, false, true, 2, row_lds);
+    else NSX_BT(true, true, true, 2, span_lds);
+#undef NSX_BT
     NSX_HIP(hipGetLastError(), "sparse backtrack launch");
     hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, lt.dense,
                        d_paths, tol, N, L);

@@ -3,6 +3,8 @@
 Distances, snapshots and backtracked paths must be bit-exact against the
 oracle's Jacobi restatement of hedetniemi_gpu.cu:47-107.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -87,6 +89,32 @@ def test_contact_graph_paths_bitexact():
     H, Hi, P = _gpu(A, L, 1e-4)
     Ho, Hio, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
     np.testing.assert_array_equal(H.view(np.uint32), Ho.view(np.uint32))
+    np.testing.assert_array_equal(P, Po)
+
+
+@pytest.mark.parametrize("L", [7, 8])
+def test_paths_unaligned_buffer_bitexact(L):
+    """A paths pointer that is not 16-byte aligned (4 bytes into a buffer)
+    takes the one-store-per-step backtrack; the 16-byte-block stores of the
+    aligned case must give the same entries, and the word before the pointer
+    stays untouched."""
+    N = 120
+    print(f"apsp unaligned paths L={L}", flush=True)
+    A = _graph(N, 0.06, seed=40 + L)
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    buf = abi.DeviceArray.from_numpy(np.full(N * L * N + 1, -7, np.int32))
+
+    class _View:
+        ptr = ctypes.c_void_p(buf.ptr.value + 4)
+
+    abi.hedetniemi(dA, N, L, 1e-4, dH, dHi, _View)
+    got = buf.numpy()
+    _, _, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
+    assert got[0] == -7
+    np.testing.assert_array_equal(got[1:], Po)
+    _, _, P = _gpu(A, L, 1e-4)
     np.testing.assert_array_equal(P, Po)
 
 
