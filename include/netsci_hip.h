@@ -194,7 +194,8 @@ int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx,
  * kernels :47-107). H_{p+1}[i][j] = min_k fl(A[j][k] + H_p[k][i]) for
  * p = 0..L-1 from H_0 = A, double-buffered (Jacobi), d_Hi[p] = H_p
  * (L*N*N floats, nullable when d_paths is null), then the backtrack of
- * :70-107 into d_paths[i][j][0..L-1] (N*L*N ints). Diagonal: paths[i][i][*] = i. */
+ * :70-107 into d_paths[i][j][0..L-1] (N*L*N ints). Diagonal: paths[i][i][*] = i.
+ * With d_paths, N <= 92680. */
 int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H,
                    float *d_Hi, int *d_paths, void *stream);
 

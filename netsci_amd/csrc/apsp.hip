@@ -355,28 +355,42 @@ struct __attribute__((aligned(8))) Span {
 // step 58.7 ms; non-temporal dword stores 109 ms; 16-byte non-temporal
 // blocks 44.5 ms (plain 16-byte stores 47.1); list starts in LDS 44.0;
 // two list groups in flight per iteration 37.8 ms; four need 77 VGPRs,
-// which halves the resident waves: 54.8 ms.
+// which halves the resident waves: 54.8 ms. Tests of a group evaluated
+// without branches (a select chain keeps the first passing slot; the
+// per-entry early exits compiled to nested exec-mask branches) 35.7 ms;
+// with that, three groups 36.6 ms and four 44.0 ms.
 constexpr int kBtThreads = 1024;
+constexpr int kMaxNodes = 92680;  // N * (N + 3) / 4 list groups fit an int
 constexpr int kLdsRowMax = 8192;
 constexpr int kRowPerLane = kLdsRowMax / kBtThreads;
 
 typedef int nsx_v4i __attribute__((ext_vector_type(4)));
 
 // First entry of one list group {q0..q3 | a0..a3} that passes the test, in
-// slot order; -1 if none.
+// slot order; -1 if none (or if `valid` is false). Branch-free: the four
+// tests are evaluated and a select chain keeps the lowest passing slot.
 template <bool kLdsRow>
 __device__ __forceinline__ int first_pass(const int4 q, const float4 a, const float *cur,
                                           const float *hrow, int j, float len, float tol,
-                                          float &af) {
+                                          bool valid, float &af) {
     const float h0 = kLdsRow ? cur[q.x] : hrow[q.x];
     const float h1 = kLdsRow ? cur[q.y] : hrow[q.y];
     const float h2 = kLdsRow ? cur[q.z] : hrow[q.z];
     const float h3 = kLdsRow ? cur[q.w] : hrow[q.w];
-    if (q.x != j && (h0 + a.x) - len < tol) { af = a.x; return q.x; }
-    if (q.y != j && (h1 + a.y) - len < tol) { af = a.y; return q.y; }
-    if (q.z != j && (h2 + a.z) - len < tol) { af = a.z; return q.z; }
-    if (q.w != j && (h3 + a.w) - len < tol) { af = a.w; return q.w; }
-    return -1;
+    const bool p0 = valid & (q.x != j) & ((h0 + a.x) - len < tol);
+    const bool p1 = valid & (q.y != j) & ((h1 + a.y) - len < tol);
+    const bool p2 = valid & (q.z != j) & ((h2 + a.z) - len < tol);
+    const bool p3 = valid & (q.w != j) & ((h3 + a.w) - len < tol);
+    int f = p3 ? q.w : -1;
+    float fa = a.w;
+    f = p2 ? q.z : f;
+    fa = p2 ? a.z : fa;
+    f = p1 ? q.y : f;
+    fa = p1 ? a.y : fa;
+    f = p0 ? q.x : f;
+    fa = p0 ? a.x : fa;
+    af = fa;
+    return f;
 }
 
 // kLdsRow: Hi[p][i][.] staged in LDS (two row buffers, N <= kLdsRowMax).
@@ -431,34 +445,38 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
             }
         }
         if (active) {
-            long long gb, ge;  // groups [gb, ge) of k's list
+            int gb, ge;  // groups [gb, ge) of k's list (N <= kMaxNodes: fits int)
             if (kLdsSpan) {
                 gb = span_lds[k];
                 ge = span_lds[k + 1];
             } else {
                 const Span sp = *reinterpret_cast<const Span *>(rows + k);
-                gb = sp.beg >> 2;
-                ge = sp.end >> 2;
+                gb = (int)(sp.beg >> 2);
+                ge = (int)(sp.end >> 2);
             }
             int found = -1;
             float af = 0.0f;
-            for (long long gi = gb; gi < ge && found < 0; gi += kGroups) {
+            for (int gi = gb; gi < ge && found < 0; gi += kGroups) {
+                // Groups past the list's end are read at the last group of
+                // the list (in bounds) and masked out.
                 int4 q[kGroups];
                 float4 a[kGroups];
 #pragma unroll
                 for (int u = 0; u < kGroups; ++u) {
-                    if (u == 0 || gi + u < ge) {
-                        q[u] = *reinterpret_cast<const int4 *>(lst + 8 * (gi + u));
-                        a[u] = *reinterpret_cast<const float4 *>(lst + 8 * (gi + u) + 4);
-                    } else {
-                        q[u] = make_int4(0, 0, 0, 0);
-                        a[u] = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(),
-                                           __builtin_inff());
-                    }
+                    const int gu = gi + u < ge ? gi + u : ge - 1;
+                    const int *grp = lst + 8 * (size_t)(unsigned)gu;
+                    q[u] = *reinterpret_cast<const int4 *>(grp);
+                    a[u] = *reinterpret_cast<const float4 *>(grp + 4);
                 }
 #pragma unroll
-                for (int u = 0; u < kGroups; ++u)
-                    if (found < 0) found = first_pass<kLdsRow>(q[u], a[u], cur, hrow, j, len, tol, af);
+                for (int u = 0; u < kGroups; ++u) {
+                    float au;
+                    const int fu = first_pass<kLdsRow>(q[u], a[u], cur, hrow, j, len, tol,
+                                                       gi + u < ge, au);
+                    const bool take = found < 0 && fu >= 0;
+                    af = take ? au : af;
+                    found = take ? fu : found;
+                }
             }
             if (found >= 0) {
                 len -= af;
@@ -662,6 +680,7 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     clear_error();
     NSX_REQUIRE(d_A && d_H && N >= 1 && L >= 1, "nsx_hedetniemi: invalid arguments");
     NSX_REQUIRE(!d_paths || d_Hi, "nsx_hedetniemi: paths need the Hi snapshot buffer");
+    NSX_REQUIRE(!d_paths || N <= kMaxNodes, "nsx_hedetniemi: paths need N <= 92680");
     if (!d_Hi) return nsx_hedetniemi_lengths(d_A, N, L, d_H, stream);
     int rc = ensure_device();
     if (rc) return rc;
