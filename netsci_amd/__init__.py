@@ -2,8 +2,9 @@
 generalized-correlation pair loop and Hedetniemi all-pairs shortest paths.
 
 The user-facing modules mirror the reference's SWIG modules:
-``netsci_amd.cuarray`` (FloatCuArray, IntCuArray) and ``netsci_amd.netcalc``
-(mutualInformation, generalizedCorrelation, hedetniemiAllShortestPaths, ...).
+``netsci_amd.cuarray`` (FloatCuArray, IntCuArray), ``netsci_amd.netcalc``
+(mutualInformation, generalizedCorrelation, hedetniemiAllShortestPaths, ...)
+and ``netsci_amd.netchem`` (Network: node coordinates from PDB + DCD).
 Both sit on ``_netsci`` (C++ CuArray / netcalc host layer, pybind11), which
 calls ``libnsx.so`` (HIP kernels behind the C ABI in include/netsci_hip.h).
 There is no pure-Python fallback: importing without the built extension

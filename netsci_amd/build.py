@@ -22,7 +22,7 @@ ARCH = "gfx950"
 HIP_SOURCES = ["nsx_runtime.hip", "ksg.hip", "ksg_d1.hip", "ksg_d2.hip", "ksg_d3.hip",
                "apsp.hip", "screener.hip", "stats.hip"]
 HOST_SOURCES = ["host_cpu.cpp"]
-PY_SOURCES = ["host/netcalc.cpp", "host/py_module.cpp"]
+PY_SOURCES = ["host/netcalc.cpp", "host/netchem.cpp", "host/py_module.cpp"]
 
 # Packed FP32 instructions stay off: hipcc would otherwise form v_pk_* ops.
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -1,6 +1,7 @@
-// Python bindings: the `cuarray` and `netcalc` surfaces of the reference's
-// SWIG modules (cuarray/swig/python_cuarray*.i, netcalc/swig/python_netcalc*.i)
-// with the same names, keyword names and numpy conventions. Exceptions map to
+// Python bindings: the `cuarray`, `netcalc` and `netchem` surfaces of the
+// reference's SWIG modules (cuarray/swig/python_cuarray*.i,
+// netcalc/swig/python_netcalc*.i, netchem/swig/python_netchem*.i) with the
+// same names, keyword names and numpy conventions. Exceptions map to
 // Python: std::invalid_argument -> ValueError, std::out_of_range -> IndexError,
 // std::runtime_error -> RuntimeError.
 #include <pybind11/numpy.h>
@@ -12,6 +13,7 @@
 
 #include "cuarray.h"
 #include "netcalc.h"
+#include "netchem.h"
 
 namespace py = pybind11;
 using namespace pybind11::literals;
@@ -142,12 +144,71 @@ void bind_cuarray(py::module_ &m, const char *name, const char *row_name) {
         });
 }
 
+// netchem/include/{atom,atoms,node,network}.h as wrapped by
+// netchem/swig/python_netchem*.i: accessor methods, Atoms iterable.
+void bind_netchem(py::module_ &m) {
+    using namespace netchem;
+    const auto ref = py::return_value_policy::reference_internal;
+    py::class_<Atom>(m, "Atom")
+        .def("index", [](const Atom &a) { return a.index; })
+        .def("serial", [](const Atom &a) { return a.serial; })
+        .def("name", [](const Atom &a) { return a.name; })
+        .def("element", [](const Atom &a) { return a.element; })
+        .def("residueName", [](const Atom &a) { return a.residueName; })
+        .def("residueId", [](const Atom &a) { return a.residueId; })
+        .def("chainId", [](const Atom &a) { return a.chainId; })
+        .def("segmentId", [](const Atom &a) { return a.segmentId; })
+        .def("occupancy", [](const Atom &a) { return a.occupancy; })
+        .def("temperatureFactor", [](const Atom &a) { return a.temperatureFactor; })
+        .def("tag", [](const Atom &a) { return a.tag; })
+        .def("mass", [](const Atom &a) { return a.mass; })
+        .def("__repr__", [](const Atom &a) {
+            return "Atom(" + std::to_string(a.index) + ", " + a.name + ", " + a.tag + ")";
+        });
+    py::class_<Atoms>(m, "Atoms")
+        .def("numAtoms", &Atoms::numAtoms)
+        .def("numUniqueTags", &Atoms::numUniqueTags)
+        .def("atoms", &Atoms::atoms, ref)
+        .def("at", &Atoms::at, "atomIndex"_a, ref)
+        .def("__len__", &Atoms::numAtoms)
+        .def("__getitem__", &Atoms::at, ref)
+        .def("__iter__", [](Atoms &a) { return py::iter(py::cast(a.atoms(), py::return_value_policy::reference)); },
+             py::keep_alive<0, 1>());
+    py::class_<Node>(m, "Node")
+        .def("index", [](const Node &n) { return n.index; })
+        .def("tag", [](const Node &n) { return n.tag; })
+        .def("numAtoms", &Node::numAtoms)
+        .def("totalMass", [](const Node &n) { return n.totalMass; })
+        .def("atoms", [](const Node &n) { return n.atoms; }, ref)
+        .def("__repr__", [](const Node &n) {
+            return "Node(" + std::to_string(n.index) + ", " + n.tag + ")";
+        });
+    py::class_<Network>(m, "Network")
+        .def(py::init<>())
+        .def("init", &Network::init, "trajectoryFile"_a, "topologyFile"_a, "firstFrame"_a,
+             "lastFrame"_a, "stride"_a = 1)
+        .def("numNodes", &Network::numNodes)
+        .def("numFrames", &Network::numFrames)
+        .def("nodeCoordinates", py::overload_cast<>(&Network::nodeCoordinates), ref)
+        .def("nodeCoordinates", py::overload_cast<const std::string &>(&Network::nodeCoordinates),
+             "nodeCoordinatesFile"_a)
+        .def("nodes", &Network::nodes, ref)
+        .def("atoms", &Network::atoms, ref)
+        .def("nodeFromAtomIndex", &Network::nodeFromAtomIndex, "atomIndex"_a, ref);
+    m.def("dcdFrameCount", [](const std::string &path) { return read_dcd_layout(path).nframes; },
+          "trajectoryFile"_a);
+    m.def("dcdAtomCount", [](const std::string &path) { return read_dcd_layout(path).natoms; },
+          "trajectoryFile"_a);
+    m.def("atomicWeight", &atomic_weight, "symbol"_a);
+}
+
 }  // namespace
 
 PYBIND11_MODULE(_netsci, m) {
     m.doc() = "MI355X engine behind the netcalc / cuarray API";
     bind_cuarray<float>(m, "FloatCuArray", "FloatCuArrayRow");
     bind_cuarray<int>(m, "IntCuArray", "IntCuArrayRow");
+    bind_netchem(m);
 
     using namespace netcalc;
     m.attr("GPU_PLATFORM") = GPU_PLATFORM;

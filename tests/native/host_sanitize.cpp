@@ -1,23 +1,30 @@
 // Host-side code under AddressSanitizer and UndefinedBehaviorSanitizer
 // (built and run by tests/test_host_sanitizers.py with g++): the CPU
 // platform's KSG (libstdc++-order selection), the prefix selection on
-// adversarial inputs, CPU Hedetniemi and adjacency, row statistics, and the
-// host side of CuArray (views, deep copies, .npy save/load, sort/argsort).
+// adversarial inputs, CPU Hedetniemi and adjacency, row statistics, the
+// host side of CuArray (views, deep copies, .npy save/load, sort/argsort),
+// and netchem's PDB / DCD readers on the golden files and on truncated,
+// byte-swapped and malformed copies (argv[1] = tests/golden/netchem).
 // Results are also checked against direct recomputations where that is
 // short. The runtime symbols the host code calls into (error string, psi
 // table, device allocator) are defined here; the device paths are not
 // exercised.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
+#include <iterator>
 #include <numeric>
 #include <random>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/netsci_hip.h"
 #include "../../netsci_amd/csrc/host/cuarray.h"
+#include "../../netsci_amd/csrc/host/netchem.h"
 
 namespace nsx {
 static std::string g_err;
@@ -173,12 +180,94 @@ static void test_cuarray() {
     CHECK(a.toDevice() == 1 && a.allocatedDevice() == 0, "toDevice without device");
 }
 
-int main() {
+static std::vector<char> slurp(const std::string &f) {
+    std::ifstream in(f, std::ios::binary);
+    return std::vector<char>(std::istreambuf_iterator<char>(in), {});
+}
+
+static void spill(const std::string &f, const std::vector<char> &b) {
+    std::ofstream(f, std::ios::binary).write(b.data(), (std::streamsize)b.size());
+}
+
+template <class F>
+static bool raises(F f) {
+    try {
+        f();
+    } catch (const std::exception &) {
+        return true;
+    }
+    return false;
+}
+
+static void test_netchem(const std::string &gold) {
+    const std::string pdb = gold + "/test.pdb", dcd = gold + "/test.dcd";
+    netchem::Network net;
+    net.init(dcd, pdb, 0, 9, 1);
+    CHECK(net.numNodes() == 290 && net.numFrames() == 10, "netchem shape");
+    CHECK(net.nodeCoordinates()->m() == 290 && net.nodeCoordinates()->n() == 30, "coords");
+    netchem::Network s3;
+    s3.init(dcd, pdb, 0, -1, 3);
+    for (int i = 0; i < 290; ++i)
+        for (int j = 0; j < 4; ++j)
+            CHECK(s3.nodeCoordinates()->get(i, j) == net.nodeCoordinates()->get(i, 3 * j),
+                  "stride 3 node %d frame %d", i, j);
+    netchem::Network pg;
+    pg.init(gold + "/protG_short.dcd", gold + "/protG_short.pdb", 5, 97, 7);
+    CHECK(pg.numNodes() == 56 && pg.numFrames() == 14, "protein G shape");
+    // Byte-swapped copy: every 4-byte word of the file reversed.
+    std::vector<char> b = slurp(dcd);
+    std::vector<char> sw = b;
+    // Words are 4 bytes except the unit-cell doubles (skipped as a unit) and
+    // the title text; reversing each 4-byte word is a valid big-endian file
+    // for everything that is read.
+    for (size_t i = 0; i + 4 <= sw.size(); i += 4) std::reverse(sw.begin() + i, sw.begin() + i + 4);
+    std::copy(b.begin() + 4, b.begin() + 8, sw.begin() + 4);  // "CORD" stays text
+    spill("swapped.dcd", sw);
+    netchem::Network be;
+    be.init("swapped.dcd", pdb, 0, 9, 1);
+    for (int e = 0; e < 290 * 30; ++e)
+        CHECK((*be.nodeCoordinates())[e] == (*net.nodeCoordinates())[e], "swapped %d", e);
+    // Truncations at every region of the header and the first frames.
+    for (size_t cut : {(size_t)0, (size_t)3, (size_t)50, (size_t)95, (size_t)200, (size_t)280,
+                       (size_t)300, (size_t)9000, (size_t)40000, b.size() - 1}) {
+        spill("cut.dcd", std::vector<char>(b.begin(), b.begin() + cut));
+        CHECK(raises([&] {
+                  netchem::Network t;
+                  t.init("cut.dcd", pdb, 0, 9, 1);
+              }),
+              "truncated DCD at %zu", cut);
+    }
+    // Corrupt record length in frame 4.
+    std::vector<char> bad = b;
+    bad[276 + 4 * 27800 + 56] ^= 0x10;
+    spill("bad.dcd", bad);
+    CHECK(raises([&] {
+              netchem::Network t;
+              t.init("bad.dcd", pdb, 0, 9, 1);
+          }),
+          "corrupt record");
+    std::ofstream("short.pdb") << "ATOM      1  CA  MET X   1      35.611  24.523\n";
+    CHECK(raises([&] {
+              netchem::Network t;
+              t.init(dcd, "short.pdb", 0, 0, 1);
+          }),
+          "short PDB record");
+    CHECK(raises([] { netchem::parse_atom_record(
+                          "ATOM      1  CA  MET X   Z      35.611  24.523  44.414  0.00  0.00"
+                          "           C",
+                          0); }),
+          "bad residue id");
+    for (const char *f : {"swapped.dcd", "cut.dcd", "bad.dcd", "short.pdb"})
+        std::remove(f);
+}
+
+int main(int argc, char **argv) {
     std::mt19937 rng(12345);
     test_ksg(rng);
     test_select_prefix(rng);
     test_apsp_and_stats(rng);
     test_cuarray();
+    if (argc > 1) test_netchem(argv[1]);
     if (failures) {
         std::fprintf(stderr, "%d failures\n", failures);
         return 1;
