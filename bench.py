@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--no-screen", action="store_true",
                     help="gc workload: skip the screened pass (screener trained on 8 other "
                          "trajectories of the same size, then all pairs through it)")
+    ap.add_argument("--method", choices=["auto", "lists", "all-frames"], default="auto",
+                    help="KSG algorithm of the plain pass (identical results)")
     ap.add_argument("--pass2", choices=["default", "plain", "symmetric"], default="default",
                     help="KSG pass 2 for the gc workload: the library default, each ordered "
                          "frame pair, or each unordered pair once (both frames credited)")
@@ -129,6 +131,8 @@ def main():
     dev_name, cus, _ = abi.device_info()
     if args.pass2 != "default":
         abi.set_config(2, 128, 4, 0, 1 if args.pass2 == "symmetric" else 0)
+    abi.set_method({"auto": abi.METHOD_AUTO, "lists": abi.METHOD_LISTS,
+                    "all-frames": abi.METHOD_ALL_FRAMES}[args.method])
     N, n, d, k = args.nodes, args.frames, 3, args.k
     X, _ = synth.calpha_trajectory(N=N, n=n, seed=1)
     ab = synth.all_pairs(N)

@@ -109,6 +109,21 @@ int nsx_ksg_last_tie_frames(long long *count);
  * launch shape is fixed, mutual_information_gpu.cu:1055-1062). */
 int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symmetric);
 
+/* Algorithm of nsx_ksg_mi (single window): method 0 = the sorted-list path
+ * whenever it applies (at most 2048 frames, finite coordinates, its tables
+ * of 4*n*n bytes per used node fit in free device memory) and the all-frames
+ * kernel otherwise; 1 = always the all-frames kernel; 2 = the list path or
+ * an error. walk_waves (1..8) and walk_batch (1..64) shape the list path's
+ * walk kernel (0 = default). Results are identical for every choice. No
+ * reference counterpart (the reference always compares all frame pairs,
+ * mutual_information_gpu.cu:62-687). */
+int nsx_ksg_set_method(int method, int walk_waves, int walk_batch);
+
+/* Which algorithm the last nsx_ksg_mi call on this thread ran (1 all-frames,
+ * 2 sorted lists) and, for the list path, the list entries walked (per
+ * list, summed over all query frames). */
+int nsx_ksg_last_method(int *method, long long *list_entries);
+
 /* ---- batched plans: one hipGraph launch per replay ----------------------
  * A plan fixes X, the windows, the pairs and the output of
  * nsx_ksg_mi_windows, owns its psi table, window table and tie scratch, and

@@ -39,6 +39,8 @@ SIGNATURES = {
     "nsx_ksg_mi_windows": [_VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I, _VP, _VP],
     "nsx_ksg_last_tie_frames": [_LLP],
     "nsx_ksg_set_config": [_I, _I, _I, _I, _I],
+    "nsx_ksg_set_method": [_I, _I, _I],
+    "nsx_ksg_last_method": [_IP, _LLP],
     "nsx_ksg_plan_create": [ctypes.POINTER(_VP), _VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I,
                             _VP, _I],
     "nsx_ksg_plan_launch": [_VP, _VP],
@@ -214,6 +216,24 @@ class KsgPlan:
             self.close()
         except Exception:
             pass
+
+
+METHOD_AUTO, METHOD_ALL_FRAMES, METHOD_LISTS = 0, 1, 2
+
+
+def set_method(method=METHOD_AUTO, walk_waves=0, walk_batch=0):
+    """Algorithm of the single-window KSG call: METHOD_AUTO (sorted lists
+    when they apply), METHOD_ALL_FRAMES, or METHOD_LISTS (lists or an error).
+    Results are identical for every choice."""
+    check(lib().nsx_ksg_set_method(int(method), int(walk_waves), int(walk_batch)), "set method")
+
+
+def last_method():
+    """(method the last ksg_mi call ran: 1 all-frames / 2 lists, list entries
+    walked per list)."""
+    m, e = ctypes.c_int(0), ctypes.c_longlong(0)
+    lib().nsx_ksg_last_method(ctypes.byref(m), ctypes.byref(e))
+    return m.value, e.value
 
 
 def last_tie_frames():

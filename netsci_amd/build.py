@@ -20,7 +20,7 @@ OBJ = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["nsx_runtime.hip", "ksg.hip", "ksg_d1.hip", "ksg_d2.hip", "ksg_d3.hip",
-               "apsp.hip", "screener.hip", "stats.hip"]
+               "ksg_walk_d1.hip", "ksg_walk_d2.hip", "ksg_walk_d3.hip", "apsp.hip", "screener.hip", "stats.hip"]
 HOST_SOURCES = ["host_cpu.cpp"]
 PY_SOURCES = ["host/netcalc.cpp", "host/netchem.cpp", "host/py_module.cpp"]
 

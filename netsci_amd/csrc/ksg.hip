@@ -22,6 +22,7 @@
 
 #include "../../include/netsci_hip.h"
 #include "ksg_kernel.h"
+#include "ksg_walk.h"
 #include "nsx_internal.h"
 
 namespace nsx {
@@ -274,16 +275,267 @@ int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &be
     return kOk;
 }
 
-// ---- eager path -------------------------------------------------------------
+// ---- list path (ksg_walk.h) -------------------------------------------------
 
+// Method: 0 picks the list path whenever it applies, 1 always runs the
+// all-frames kernel, 2 requires the list path.
+int g_method = 0;
+int g_walk_wpb = 4;    // waves per walk workgroup
+int g_walk_batch = 16; // finished lanes a wave waits for before a refill round
 thread_local long long g_last_ties = 0;
+thread_local int g_last_method = 0;         // 1 all-frames, 2 lists
+thread_local long long g_last_entries = 0;  // list entries walked (per list)
+
+ksgw::BuildFn build_for(int d, int width) {
+    switch (d) {
+        case 1: return ksgw::build_d1(width);
+        case 2: return ksgw::build_d2(width);
+        case 3: return ksgw::build_d3(width);
+        default: return nullptr;
+    }
+}
+
+ksgw::WalkFn walk_for(int d, int k) {
+    switch (d) {
+        case 1: return ksgw::walk_d1(k);
+        case 2: return ksgw::walk_d2(k);
+        case 3: return ksgw::walk_d3(k);
+        default: return nullptr;
+    }
+}
+
+// mm[0] = min, mm[1] = max over the 2P node indices.
+__global__ void ab_range_kernel(const int *ab, long long m, int *mm) {
+    int lo = 0x7fffffff, hi = -1;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < m;
+         t += (long long)gridDim.x * blockDim.x) {
+        lo = min(lo, ab[t]);
+        hi = max(hi, ab[t]);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo = min(lo, __shfl_xor(lo, off));
+        hi = max(hi, __shfl_xor(hi, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(mm, lo);
+        atomicMax(mm + 1, hi);
+    }
+}
+
+__global__ void mark_nodes_kernel(const int *ab, long long m, int *used) {
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < m;
+         t += (long long)gridDim.x * blockDim.x)
+        used[ab[t]] = 1;
+}
+
+// One workgroup of 1024 lanes: slot_of[node] = rank of node among the used
+// ones (-1 if unused), node_of[slot] = node, *count = number of used nodes.
+__global__ __launch_bounds__(1024) void slot_scan_kernel(const int *used, int N, int *slot_of,
+                                                         int *node_of, int *count) {
+    __shared__ int wave_tot[16];
+    __shared__ int carry;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < N; c0 += 1024) {
+        const int v = c0 + (int)threadIdx.x;
+        const bool u = v < N && used[v] != 0;
+        const unsigned long long b = __ballot(u);
+        const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+        if (lane == 0) wave_tot[w] = __popcll(b);
+        __syncthreads();
+        int before = carry;
+        for (int x = 0; x < w; ++x) before += wave_tot[x];
+        if (v < N) {
+            const int s = before + below;
+            slot_of[v] = u ? s : -1;
+            if (u) node_of[s] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int x = 0; x < 16; ++x) tot += wave_tot[x];
+            carry += tot;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = carry;
+}
+
+// Runs the whole problem on the list path when it applies (one window of at
+// most ksgw::kMaxFrames frames, tables within the free device memory, finite
+// coordinates); otherwise returns kOk with *ran == false.
+int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_out,
+              hipStream_t stream, bool *ran) {
+    *ran = false;
+    const int off = pb.host_win[0], n = pb.host_win[1];
+    const bool eligible = pb.R == 1 && n <= ksgw::kMaxFrames;
+    if (g_method == 1) return kOk;
+    if (!eligible) {
+        NSX_REQUIRE(g_method != 2, "ksg: the list path needs one window of at most 2048 frames");
+        return kOk;
+    }
+    const ksgw::WalkFn walk = walk_for(pb.d, pb.k);
+    const int width = ksgw::build_width(n);
+    const ksgw::BuildFn buildk = build_for(pb.d, width);
+    NSX_REQUIRE(walk && buildk, "ksg: no list-path kernel for (d, k)");
+    const float *X = pb.X + off;
+    const long long m2 = 2 * pb.P;
+
+    // Used nodes and their table slots.
+    int *mm = (int *)workspace(kWsListStats, 64);
+    if (!mm) return kHipError;
+    int rc = fill_u32(mm, 0x7fffffffu, 1, stream);
+    if (!rc) rc = fill_u32(mm + 1, 0xffffffffu, 1, stream);
+    if (rc) return rc;
+    const long long rblocks = std::min<long long>((m2 + 255) / 256, 1024);
+    hipLaunchKernelGGL(ab_range_kernel, dim3((unsigned)rblocks), dim3(256), 0, stream, pb.ab, m2,
+                       mm);
+    NSX_HIP(hipGetLastError(), "ab range launch");
+    int h_mm[2] = {0, 0};
+    NSX_HIP(hipMemcpyAsync(h_mm, mm, sizeof(h_mm), hipMemcpyDeviceToHost, stream),
+            "ab range readback");
+    NSX_HIP(hipStreamSynchronize(stream), "ab range sync");
+    NSX_REQUIRE(h_mm[0] >= 0, "ksg: negative node index in ab");
+    const int N = h_mm[1] + 1;
+    int *map = (int *)workspace(kWsListMap, ((size_t)3 * N + 16) * sizeof(int));
+    if (!map) return kHipError;
+    int *used = map, *slot_of = map + N, *node_of = map + 2 * N, *d_count = map + 3 * N;
+    rc = fill_u32(used, 0u, (size_t)N, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(mark_nodes_kernel, dim3((unsigned)rblocks), dim3(256), 0, stream, pb.ab,
+                       m2, used);
+    NSX_HIP(hipGetLastError(), "mark nodes launch");
+    hipLaunchKernelGGL(slot_scan_kernel, dim3(1), dim3(1024), 0, stream, used, N, slot_of,
+                       node_of, d_count);
+    NSX_HIP(hipGetLastError(), "slot scan launch");
+    int U = 0;
+    NSX_HIP(hipMemcpyAsync(&U, d_count, sizeof(int), hipMemcpyDeviceToHost, stream),
+            "slot count readback");
+    NSX_HIP(hipStreamSynchronize(stream), "slot scan sync");
+
+    // Tables: ord and rank, U x n rows of rs entries each.
+    const long long rs = ((long long)n + 7) & ~7LL;
+    const size_t table_bytes = (size_t)U * n * rs * sizeof(uint16_t);
+    size_t free_b = 0, total_b = 0;
+    NSX_HIP(hipMemGetInfo(&free_b, &total_b), "memory info");
+    const double budget = 0.85 * (double)free_b - 512.0 * (1 << 20);
+    if ((double)(2 * table_bytes) > budget) {
+        NSX_REQUIRE(g_method != 2, "ksg: the list-path tables do not fit in device memory");
+        return kOk;
+    }
+    uint16_t *tables = nullptr;
+    NSX_HIP(hipMalloc(&tables, 2 * table_bytes + 64), "list tables");
+    struct Free {
+        void *p;
+        ~Free() { (void)hipFree(p); }
+    } free_tables{tables};
+    uint16_t *ord = tables, *rank = tables + table_bytes / sizeof(uint16_t);
+
+    unsigned *bad = (unsigned *)(mm + 4);
+    rc = fill_u32(bad, 0u, 1, stream);
+    if (rc) return rc;
+    ksgw::BuildArgs ba;
+    ba.X = X;
+    ba.ld = pb.ld;
+    ba.dstride = pb.dstride;
+    ba.n = n;
+    ba.rs = rs;
+    ba.node_of = node_of;
+    ba.groups = (int)std::max<long long>(
+        1, std::min<long long>((n + ksgw::kBuildWaves - 1) / ksgw::kBuildWaves,
+                               (8LL * cu_count() + U - 1) / std::max(U, 1)));
+    ba.ord = ord;
+    ba.rank = rank;
+    ba.bad = bad;
+    const size_t blds = ksgw::build_lds_bytes(pb.d, n, rs);
+    hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)U * ba.groups)),
+                       dim3(ksgw::kWave * ksgw::kBuildWaves), blds, stream, ba);
+    NSX_HIP(hipGetLastError(), "list build launch");
+    unsigned h_bad = 0;
+    NSX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, stream),
+            "finite check readback");
+    NSX_HIP(hipStreamSynchronize(stream), "list build sync");
+    if (h_bad) {
+        NSX_REQUIRE(g_method != 2, "ksg: the list path needs finite coordinates");
+        return kOk;
+    }
+
+    // Walk.
+    const int wpb = g_walk_wpb;
+    const size_t wlds = ksgw::walk_lds_bytes(pb.d, n);
+    int per_cu = 0;
+    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)walk,
+                                                         ksgw::kWave * wpb, wlds),
+            "walk occupancy");
+    per_cu = std::max(per_cu, 1);
+    const long long grid = std::max<long long>(
+        1, std::min<long long>(pb.P, (long long)cu_count() * per_cu));
+    unsigned *cnt = (unsigned *)workspace(kWsListCnt, (size_t)grid * n * sizeof(unsigned));
+    KeyIdx *scratch =
+        (KeyIdx *)workspace(kWsTieScratch, (size_t)grid * wpb * n * sizeof(KeyIdx));
+    unsigned long long *stats = (unsigned long long *)(mm + 8);
+    if (!cnt || !scratch) return kHipError;
+    rc = fill_u32(stats, 0u, 4, stream);
+    if (rc) return rc;
+    ksgw::WalkArgs wa;
+    wa.X = X;
+    wa.ld = pb.ld;
+    wa.dstride = pb.dstride;
+    wa.ab = pb.ab;
+    wa.P = pb.P;
+    wa.n = n;
+    wa.k = pb.k;
+    wa.psi = d_psi;
+    wa.inv_k = (float)(1.0 / (float)pb.k);
+    wa.mi_out = mi_out;
+    wa.counts_out = counts_out;
+    wa.ord = ord;
+    wa.rank = rank;
+    wa.rs = rs;
+    wa.slot_of = slot_of;
+    wa.scratch = scratch;
+    wa.cnt = cnt;
+    wa.stats = stats;
+    wa.wpb = wpb;
+    wa.batch = g_walk_batch;
+    hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream, wa);
+    NSX_HIP(hipGetLastError(), "walk launch");
+    unsigned long long h_stats[2] = {0, 0};
+    NSX_HIP(hipMemcpyAsync(h_stats, stats, sizeof(h_stats), hipMemcpyDeviceToHost, stream),
+            "walk stats readback");
+    NSX_HIP(hipStreamSynchronize(stream), "walk sync");
+    *ran = true;
+    g_last_method = 2;
+    g_last_entries = (long long)h_stats[1];
+    g_last_ties = (long long)h_stats[0];
+    return kOk;
+}
+
+// ---- eager path -------------------------------------------------------------
 
 int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t stream) {
     NSX_REQUIRE(mi_out, "ksg: null output pointer");
     int rc = ensure_device();
     if (rc) return rc;
     g_last_ties = 0;
+    g_last_method = 0;
+    g_last_entries = 0;
     if (pb.P == 0) return kOk;
+    if (g_method != 1 && pb.R == 1) {
+        std::vector<float> psi(pb.nmax + 1);
+        psi_table(psi.data(), pb.nmax);
+        float *d_psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));
+        if (!d_psi) return kHipError;
+        NSX_HIP(hipMemcpyAsync(d_psi, psi.data(), psi.size() * sizeof(float),
+                               hipMemcpyHostToDevice, stream), "psi upload");
+        bool ran = false;
+        rc = run_lists(pb, d_psi, mi_out, counts_out, stream, &ran);
+        if (rc || ran) return rc;
+    }
+    g_last_method = 1;
     const Config c = config_for(pb);
     Shape s;
     rc = shape_for(pb, c, (long long)pb.R * pb.P, s);
@@ -375,6 +627,25 @@ int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
 
 int nsx_ksg_last_tie_frames(long long *count) {
     if (count) *count = g_last_ties;
+    return kOk;
+}
+
+int nsx_ksg_set_method(int method, int walk_waves, int walk_batch) {
+    clear_error();
+    NSX_REQUIRE(method >= 0 && method <= 2, "nsx_ksg_set_method: method must be 0, 1 or 2");
+    NSX_REQUIRE(walk_waves >= 0 && walk_waves <= ksgw::kMaxWalkWaves,
+                "nsx_ksg_set_method: walk_waves must be 0..8");
+    NSX_REQUIRE(walk_batch >= 0 && walk_batch <= 64, "nsx_ksg_set_method: walk_batch must be 0..64");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_method = method;
+    g_walk_wpb = walk_waves ? walk_waves : 4;
+    g_walk_batch = walk_batch ? walk_batch : 16;
+    return kOk;
+}
+
+int nsx_ksg_last_method(int *method, long long *list_entries) {
+    if (method) *method = g_last_method;
+    if (list_entries) *list_entries = g_last_entries;
     return kOk;
 }
 

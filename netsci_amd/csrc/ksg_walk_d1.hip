@@ -1,0 +1,12 @@
+// List-path KSG instances (build and walk kernels) for d = 1, one
+// translation unit per dimension so that they compile in parallel.
+#include "ksg_walk.h"
+
+namespace nsx {
+namespace ksgw {
+
+BuildFn build_d1(int width) { return build_lookup<1>(width); }
+WalkFn walk_d1(int k) { return walk_lookup<1>(k); }
+
+}  // namespace ksgw
+}  // namespace nsx

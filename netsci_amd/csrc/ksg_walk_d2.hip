@@ -1,0 +1,12 @@
+// List-path KSG instances (build and walk kernels) for d = 2, one
+// translation unit per dimension so that they compile in parallel.
+#include "ksg_walk.h"
+
+namespace nsx {
+namespace ksgw {
+
+BuildFn build_d2(int width) { return build_lookup<2>(width); }
+WalkFn walk_d2(int k) { return walk_lookup<2>(k); }
+
+}  // namespace ksgw
+}  // namespace nsx

@@ -1,0 +1,12 @@
+// List-path KSG instances (build and walk kernels) for d = 3, one
+// translation unit per dimension so that they compile in parallel.
+#include "ksg_walk.h"
+
+namespace nsx {
+namespace ksgw {
+
+BuildFn build_d3(int width) { return build_lookup<3>(width); }
+WalkFn walk_d3(int k) { return walk_lookup<3>(k); }
+
+}  // namespace ksgw
+}  // namespace nsx

@@ -76,7 +76,10 @@ enum WorkspaceSlot : int {
     kWsApspRowsA = 11,
     kWsApspCandA = 12,
     kWsApspG = 13,
-    kWsCount = 16,
+    kWsListMap = 14,
+    kWsListCnt = 15,
+    kWsListStats = 16,
+    kWsCount = 20,
 };
 
 // Fill `count` 32-bit words at d_ptr with `value` on `stream` (a kernel, so

@@ -24,6 +24,7 @@ def test_every_launch_shape_is_bitexact():
     dX = abi.DeviceArray.from_numpy(X)
     dab = abi.DeviceArray.from_numpy(ab)
     dmi = abi.DeviceArray((len(ab),), np.float32)
+    abi.set_method(abi.METHOD_ALL_FRAMES)
     try:
         for q, c, w, s in SHAPES:
             print(f"shape qpl={q} tile={c} wpb={w} symmetric={s}", flush=True)
@@ -32,6 +33,7 @@ def test_every_launch_shape_is_bitexact():
             np.testing.assert_array_equal(dmi.numpy(), want, err_msg=f"shape {(q, c, w, s)}")
     finally:
         abi.set_config(0)
+        abi.set_method(abi.METHOD_AUTO)
 
 
 def test_plan_replays_match_eager():

@@ -24,13 +24,22 @@ def test_ksg_twice_with_ties_and_shapes():
     dab = abi.DeviceArray.from_numpy(ab)
     runs = []
     try:
+        # All-frames kernel in several launch shapes, then the sorted-list
+        # path twice and with another walk shape.
+        abi.set_method(abi.METHOD_ALL_FRAMES)
         for shape in [(2, 128, 4, 0, 0), (2, 128, 4, 0, 0), (1, 64, 8, 0, 1), (2, 256, 1, 2, 0)]:
             abi.set_config(*shape)
             dmi = abi.DeviceArray((len(ab),), np.float32)
             abi.ksg_mi(dX, X.shape[1], n, d, n, dab, len(ab), k, dmi)
             runs.append((dmi.numpy(), abi.last_tie_frames()))
+        for waves, batch in [(4, 16), (4, 16), (8, 5)]:
+            abi.set_method(abi.METHOD_LISTS, waves, batch)
+            dmi = abi.DeviceArray((len(ab),), np.float32)
+            abi.ksg_mi(dX, X.shape[1], n, d, n, dab, len(ab), k, dmi)
+            runs.append((dmi.numpy(), abi.last_tie_frames()))
     finally:
         abi.set_config(0)
+        abi.set_method(abi.METHOD_AUTO)
     assert runs[0][1] > 0
     for mi, ties in runs[1:]:
         np.testing.assert_array_equal(mi, runs[0][0])

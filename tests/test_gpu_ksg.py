@@ -14,13 +14,19 @@ from tests.conftest import curve_inputs
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=[0, 1], ids=["pass2", "sympass2"])
-def pass2_variant(request):
-    """Every test runs with the plain pass 2 (each ordered frame pair) and
-    with the symmetric one (each unordered pair once, both frames credited);
-    both must give the oracle's bits."""
-    abi.set_config(2, 128, 4, 0, request.param)
+@pytest.fixture(autouse=True, params=["lists", "pass2", "sympass2"])
+def method_variant(request):
+    """Every test runs on the sorted-list path (where it applies) and on the
+    all-frames kernel with the plain pass 2 (each ordered frame pair) and the
+    symmetric one (each unordered pair once, both frames credited); all must
+    give the oracle's bits."""
+    if request.param == "lists":
+        abi.set_method(abi.METHOD_AUTO)
+    else:
+        abi.set_method(abi.METHOD_ALL_FRAMES)
+        abi.set_config(2, 128, 4, 0, int(request.param == "sympass2"))
     yield request.param
+    abi.set_method(abi.METHOD_AUTO)
     abi.set_config(0)
 
 
@@ -224,3 +230,95 @@ def test_invalid_arguments_raise():
         abi.ksg_mi(dX, 100, 100, 1, 100, dab, 1, 17, dmi)
     with pytest.raises(RuntimeError, match="d = 1, 2 or 3"):
         abi.ksg_mi(dX, 100, 25, 4, 25, dab, 1, 4, dmi)
+
+
+# ---- sorted-list path specifics ------------------------------------------------
+
+
+def _check_pairs(X, ab, k, n, d, expect_method=None):
+    mi, cnt, ties = _gpu_counts(X, ab, k, n, d)
+    if expect_method is not None:
+        assert abi.last_method()[0] == expect_method
+    for p, (a, b) in enumerate(ab):
+        na, nb, _, _ = oracle.ksg_counts(X[a, : d * n], X[b, : d * n], k, n, d)
+        np.testing.assert_array_equal(cnt[p, :, 0], na)
+        np.testing.assert_array_equal(cnt[p, :, 1], nb)
+        assert mi[p] == np.float32(oracle.mi(X[a, : d * n], X[b, : d * n], k, n, d))
+    return ties
+
+
+@pytest.mark.parametrize("n,k", [(7, 6), (8, 6), (9, 4), (16, 15), (17, 16), (511, 6), (512, 6),
+                                 (513, 6), (1024, 3), (1025, 6), (2047, 6), (2048, 6)])
+def test_list_path_sizes(method_variant, n, k):
+    """Row strides and sort widths at their boundaries (8, 16 or 32 keys per
+    lane; 8-entry list blocks), the smallest n = k + 1 and the largest n."""
+    d = 3
+    _flush(f"list sizes n={n} k={k} ({method_variant})")
+    X3, _ = synth.calpha_trajectory(N=4, n=n, seed=100 + n)
+    ab = np.array([[0, 1], [2, 3], [1, 1]], np.int32)
+    _check_pairs(X3, ab, k, n, d, 2 if method_variant == "lists" else 1)
+
+
+def test_list_path_limits_and_fallback(method_variant):
+    """Above 2048 frames, or with a non-finite coordinate, the automatic
+    choice runs the all-frames kernel; METHOD_LISTS raises instead."""
+    if method_variant != "lists":
+        pytest.skip("method selection is exercised in the lists variant")
+    d, k = 3, 6
+    _flush("list fallback n=2049")
+    X3, _ = synth.calpha_trajectory(N=2, n=2049, seed=5)
+    _check_pairs(X3, np.array([[0, 1]], np.int32), k, 2049, d, 1)
+    abi.set_method(abi.METHOD_LISTS)
+    with pytest.raises(RuntimeError, match="2048"):
+        _gpu_counts(X3, np.array([[0, 1]], np.int32), k, 2049, d)
+    _flush("list fallback non-finite")
+    X3, _ = synth.calpha_trajectory(N=2, n=300, seed=6)
+    X3[1, 17] = np.inf
+    with pytest.raises(RuntimeError, match="finite"):
+        _gpu_counts(X3, np.array([[0, 1]], np.int32), k, 300, d)
+    abi.set_method(abi.METHOD_AUTO)
+    mi, _, _ = _gpu_counts(X3, np.array([[0, 0]], np.int32), k, 300, d)
+    assert abi.last_method()[0] == 2  # node 1 (the non-finite row) is not used
+    mi, _, _ = _gpu_counts(X3, np.array([[0, 1]], np.int32), k, 300, d)
+    assert abi.last_method()[0] == 1
+
+
+@pytest.mark.parametrize("waves,batch", [(1, 1), (2, 64), (8, 16), (4, 3)])
+def test_list_path_walk_shapes(method_variant, waves, batch):
+    """Waves per walk workgroup and the refill batch change the schedule,
+    never the bits; heavy ties exercise the cooperative tie path."""
+    if method_variant != "lists":
+        pytest.skip("walk shapes only apply to the list path")
+    n, d, k = 600, 3, 6
+    _flush(f"walk shape waves={waves} batch={batch}")
+    rng = np.random.default_rng(41)
+    X = (np.round(rng.standard_normal((5, d * n)) * 3) / 3).astype(np.float32)
+    ab = np.array([[0, 1], [2, 3], [4, 0], [3, 3]], np.int32)
+    abi.set_method(abi.METHOD_LISTS, waves, batch)
+    ties = _check_pairs(X, ab, k, n, d, 2)
+    assert ties > 0
+
+
+def test_list_path_sparse_node_indices(method_variant):
+    """Pairs over a few scattered rows of a tall X: the tables cover only the
+    used nodes (slot map), and entries walked are reported."""
+    if method_variant != "lists":
+        pytest.skip("node slots only exist on the list path")
+    n, d, k = 256, 3, 6
+    _flush("sparse node indices")
+    X, _ = synth.calpha_trajectory(N=1200, n=n, seed=12)
+    ab = np.array([[1199, 3], [3, 700], [700, 1199], [5, 5]], np.int32)
+    _check_pairs(X, ab, k, n, d, 2)
+    method, entries = abi.last_method()
+    assert method == 2 and 0 < entries <= len(ab) * n * n
+
+
+@pytest.mark.parametrize("d", [1, 2])
+def test_list_path_low_dimensions(method_variant, d):
+    """d = 1 and d = 2 with ties (coarse grid) on every variant."""
+    n, k = 700, 4
+    _flush(f"low dimensions d={d} ({method_variant})")
+    rng = np.random.default_rng(60 + d)
+    X = (np.round(rng.standard_normal((3, d * n)) * 8) / 8).astype(np.float32)
+    ab = np.array([[0, 1], [1, 2], [2, 0]], np.int32)
+    _check_pairs(X, ab, k, n, d, 2 if method_variant == "lists" else 1)
