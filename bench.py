@@ -61,6 +61,10 @@ def parse():
                          "trajectories of the same size, then all pairs through it)")
     ap.add_argument("--method", choices=["auto", "lists", "all-frames"], default="auto",
                     help="KSG algorithm of the plain pass (identical results)")
+    ap.add_argument("--walk-waves", type=int, default=0,
+                    help="list path: waves per walk workgroup (0 = default)")
+    ap.add_argument("--walk-batch", type=int, default=0,
+                    help="list path: finished lanes per refill round (0 = default)")
     ap.add_argument("--pass2", choices=["default", "plain", "symmetric"], default="default",
                     help="KSG pass 2 for the gc workload: the library default, each ordered "
                          "frame pair, or each unordered pair once (both frames credited)")
@@ -132,7 +136,8 @@ def main():
     if args.pass2 != "default":
         abi.set_config(2, 128, 4, 0, 1 if args.pass2 == "symmetric" else 0)
     abi.set_method({"auto": abi.METHOD_AUTO, "lists": abi.METHOD_LISTS,
-                    "all-frames": abi.METHOD_ALL_FRAMES}[args.method])
+                    "all-frames": abi.METHOD_ALL_FRAMES}[args.method],
+                   args.walk_waves, args.walk_batch)
     N, n, d, k = args.nodes, args.frames, 3, args.k
     X, _ = synth.calpha_trajectory(N=N, n=n, seed=1)
     ab = synth.all_pairs(N)
@@ -166,6 +171,7 @@ def main():
     abi.sync()
     wall = time.perf_counter() - t0
     ties = abi.last_tie_frames()
+    method, list_entries = abi.last_method()
 
     ms_per_step = 1e3 * wall / args.steps
     value = P * args.steps / wall
@@ -224,6 +230,8 @@ def main():
                      "work": f"(8d+2)*n*(n-1) = {work_per_pair} FP32 VALU lane-ops per pair"},
         "cpu_baseline": cpu,
         "tie_frames": ties,
+        "ksg_method": {1: "all-frames", 2: "sorted lists"}.get(method, "none"),
+        "list_entries_per_list": list_entries,
         "screened": screened,
     }
     print(json.dumps(out), flush=True)

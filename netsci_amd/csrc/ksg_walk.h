@@ -34,11 +34,16 @@
 //     (32 per lane, cross-lane stages by ds_bpermute); ranks come from a
 //     suffix minimum over the ends of equal-key runs;
 //   * walk: a workgroup owns one node pair at a time with both nodes'
-//     coordinates in LDS (gathered at random frames); its lanes take query
-//     frames from an LDS counter, each lane walking its own query. Lanes whose
-//     walk ended wait until `batch` of them have (or no lane walks), then the
-//     wave processes them together (eps, ties, rank lookups) and hands out new
-//     queries, so that per-query work runs densely;
+//     coordinates in LDS, one 2d-float record per frame (gathered at random
+//     frames with 8-byte loads); its lanes take query
+//     frames from an LDS counter, each lane walking its own query in bodies
+//     of 8 steps (queries start at a body boundary, so every body reads one
+//     aligned 8-entry block per list at fixed register positions, and the
+//     block two ahead is requested one body before it is needed). Lanes
+//     whose walk ended wait until `batch` of them have (or no lane walks),
+//     then the wave processes them together (eps, ties, rank lookups, whose
+//     counts are stored one round later) and hands out new queries, so that
+//     per-query work runs densely;
 //   * the workgroup's counts go to a per-workgroup buffer and one wave sums
 //     psi[n_x] + psi[n_y] in frame order, exactly as the reference's epilogue.
 // Coordinates must be finite (the host checks and otherwise takes
@@ -115,7 +120,6 @@ WalkFn walk_d3(int k);
 
 #if defined(__HIPCC__)
 
-using ksg::heap_insert;
 using ksg::marg;
 
 __device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
@@ -251,11 +255,55 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
     }
 }
 
+// Insert (c, j) into the ascending h[0..K] without branches: every slot
+// is two compares, a med3 and two selects. c >= h[K] (+inf for a rejected
+// candidate) leaves everything unchanged.
+template <int K>
+__device__ __forceinline__ void insert_flat(float (&h)[K + 1], int (&id)[K + 1], float c, int j) {
+#pragma unroll
+    for (int m = K; m >= 1; --m) {
+        const int t = c < h[m] ? j : id[m];
+        id[m] = c < h[m - 1] ? id[m - 1] : t;
+        h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+    }
+    id[0] = c < h[0] ? j : id[0];
+    h[0] = fminf(c, h[0]);
+}
+
+// (D, j) as one signed 64-bit value ordered like the list's (D, j) order:
+// D >= 0 is finite, so its bits compare like the float; -1 sorts first.
+__device__ __forceinline__ long long lex(float key, int j) {
+    return (long long)(((unsigned long long)__float_as_uint(key) << 32) | (unsigned)j);
+}
+
+__device__ __forceinline__ float lex_key(long long v) {
+    return __uint_as_float((unsigned)((unsigned long long)v >> 32));
+}
+
+// Frame j of the pair's LDS record array: 2D floats per frame (node a's D
+// coordinates, then node b's), read as D 8-byte LDS loads.
+template <int D>
+__device__ __forceinline__ void fetch(const float *rec, int j, float (&va)[D], float (&vb)[D]) {
+    const float2 *r = reinterpret_cast<const float2 *>(rec + 2 * D * j);
+    float v[2 * D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) {
+        const float2 w = r[t];
+        v[2 * t] = w.x;
+        v[2 * t + 1] = w.y;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        va[d] = v[d];
+        vb[d] = v[D + d];
+    }
+}
+
 __device__ __forceinline__ uint4 load_block(const uint16_t *p) {
     return *reinterpret_cast<const uint4 *>(p);
 }
 
-// Entry q (0..7) of an 8-entry block.
+// Entry q (0..7) of an 8-entry block (q is a compile-time index in the walk).
 __device__ __forceinline__ int block_entry(const uint4 &b, int q) {
     const unsigned w = (q & 4) ? ((q & 2) ? b.w : b.z) : ((q & 2) ? b.y : b.x);
     return (int)((w >> ((q & 1) * 16)) & 0xffffu);
@@ -265,8 +313,7 @@ template <int D, int K>
 __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int n = a.n;
-    const float *ca = lds;          // node a: [D][n]
-    const float *cb = lds + D * n;  // node b: [D][n]
+    const float *rec = lds;  // [n][2D]: frame j's node-a then node-b coordinates
     int *counter = reinterpret_cast<int *>(lds + 2 * D * n);
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -287,8 +334,8 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
         __syncthreads();  // the previous pair's readers are done
         for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
             const int d = t / n, f = t - d * n;
-            lds[t] = xa[d * a.dstride + f];
-            lds[D * n + t] = xb[d * a.dstride + f];
+            lds[2 * D * f + d] = xa[d * a.dstride + f];
+            lds[2 * D * f + D + d] = xb[d * a.dstride + f];
         }
         if (threadIdx.x == 0) *counter = 0;
         __syncthreads();
@@ -296,13 +343,22 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
         // Lane state: 0 wants a query, 1 walks, 2 no queries left.
         int state = 0;
         bool res = false;  // a finished walk waits for its counts
-        int i = 0, pos = 0, lja = -1, ljb = -1;
-        float la = -1.0f, lb = -1.0f;
+        int i = 0, pos = 0;
+        // Last walked entry of each list as lex(D, j); -1: none yet (its key
+        // reads as NaN, so the stop test below is false).
+        long long lastA = -1, lastB = -1;
         float qa[D], qb[D], h[K + 1];
         int id[K + 1];
-        uint4 bufA = {0, 0, 0, 0}, bufB = {0, 0, 0, 0}, nxtA = {0, 0, 0, 0},
+        // List blocks: cur holds entries [pos, pos + 8) of each list, nxt
+        // the 8 after them (pos is a multiple of 8 at every body start).
+        uint4 curA = {0, 0, 0, 0}, curB = {0, 0, 0, 0}, nxtA = {0, 0, 0, 0},
               nxtB = {0, 0, 0, 0};
         const uint16_t *rowA = ordA, *rowB = ordB;
+        // Counts of the previous refill round, stored one round later so
+        // that the rank lookups' latency is not waited for.
+        bool pend = false;
+        int pend_i = 0;
+        unsigned pend_x = 1u, pend_y = 1u;
 #pragma unroll
         for (int d = 0; d < D; ++d) qa[d] = qb[d] = 0.0f;
 #pragma unroll
@@ -315,7 +371,10 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
             const unsigned long long want = __ballot(state == 0);
             const unsigned long long walking = __ballot(state == 1);
             if (want && (__popcll(want) >= (unsigned)a.batch || walking == 0)) {
-                // (1) counts of the finished walks.
+                // (0) counts looked up in the previous round.
+                if (pend) cnt[pend_i] = (pend_x - 1u) | ((pend_y - 1u) << 16);
+                pend = false;
+                // (1) eps of the finished walks.
                 const bool tie = res && (h[K - 1] == h[K]);
                 int jx = 0, jy = 0;
                 if (res && !tie) {
@@ -323,11 +382,7 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
 #pragma unroll
                     for (int m = 0; m < K; ++m) {
                         float va[D], vb[D];
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            va[d] = ca[d * n + id[m]];
-                            vb[d] = cb[d * n + id[m]];
-                        }
+                        fetch<D>(rec, id[m], va, vb);
                         const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
                         jx = da > ex ? id[m] : jx;
                         ex = fmaxf(ex, da);
@@ -349,11 +404,7 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
                     }
                     for (int j = lane; j < n; j += kWave) {
                         float va[D], vb[D];
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            va[d] = ca[d * n + j];
-                            vb[d] = cb[d * n + j];
-                        }
+                        fetch<D>(rec, j, va, vb);
                         KeyIdx r;
                         r.key = fmaxf(marg<D>(tqa, va), marg<D>(tqb, vb));
                         r.idx = j;
@@ -367,11 +418,7 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
                         for (int m = 1; m <= K; ++m) {
                             const int j = scratch[m].idx;
                             float va[D], vb[D];
-#pragma unroll
-                            for (int d = 0; d < D; ++d) {
-                                va[d] = ca[d * n + j];
-                                vb[d] = cb[d * n + j];
-                            }
+                            fetch<D>(rec, j, va, vb);
                             const float da = marg<D>(tqa, va), db = marg<D>(tqb, vb);
                             rjx = da > ex ? j : rjx;
                             ex = fmaxf(ex, da);
@@ -387,9 +434,10 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
                     tm &= tm - 1;
                 }
                 if (res) {
-                    const unsigned rx = rankA[(long long)i * rs + jx];
-                    const unsigned ry = rankB[(long long)i * rs + jy];
-                    cnt[i] = (rx - 1u) | ((ry - 1u) << 16);
+                    pend_x = rankA[(long long)i * rs + jx];
+                    pend_y = rankB[(long long)i * rs + jy];
+                    pend_i = i;
+                    pend = true;
                     steps += (unsigned)pos;
                 }
                 res = false;
@@ -404,23 +452,18 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
                     const int qi = base + below;
                     if (qi < n) {
                         i = qi;
-#pragma unroll
-                        for (int d = 0; d < D; ++d) {
-                            qa[d] = ca[d * n + i];
-                            qb[d] = cb[d * n + i];
-                        }
+                        fetch<D>(rec, i, qa, qb);
 #pragma unroll
                         for (int m = 0; m <= K; ++m) {
                             h[m] = __builtin_inff();
                             id[m] = 0;
                         }
                         pos = 0;
-                        la = lb = -1.0f;
-                        lja = ljb = -1;
+                        lastA = lastB = -1;
                         rowA = ordA + (long long)i * rs;
                         rowB = ordB + (long long)i * rs;
-                        bufA = load_block(rowA);
-                        bufB = load_block(rowB);
+                        curA = load_block(rowA);
+                        curB = load_block(rowB);
                         if (rs > 8) {
                             nxtA = load_block(rowA + 8);
                             nxtB = load_block(rowB + 8);
@@ -432,48 +475,48 @@ __global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) 
                 }
             }
             if (__ballot(state == 1) == 0) break;
-            if (state == 1) {
-                // One entry of each list per step.
-                const int q = pos & 7;
-                const int ja = block_entry(bufA, q), jb = block_entry(bufB, q);
-                ++pos;
-                if ((pos & 7) == 0 && pos < n) {
-                    bufA = nxtA;
-                    bufB = nxtB;
-                    if (pos + 8 < rs) {
-                        nxtA = load_block(rowA + pos + 8);
-                        nxtB = load_block(rowB + pos + 8);
+            // A body: 8 steps, one entry of each list per step. The block
+            // two ahead is requested first and used one body later.
+            uint4 tA = nxtA, tB = nxtB;
+            if (state == 1 && pos + 16 < rs) {
+                tA = load_block(rowA + pos + 16);
+                tB = load_block(rowB + pos + 16);
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                if (state == 1) {
+                    const int ja = block_entry(curA, s), jb = block_entry(curB, s);
+                    ++pos;
+                    float ua[D], ub[D], va[D], vb[D];
+                    fetch<D>(rec, ja, ua, ub);
+                    fetch<D>(rec, jb, va, vb);
+                    const float daA = marg<D>(qa, ua), dbA = marg<D>(qb, ub);
+                    const float daB = marg<D>(qa, va), dbB = marg<D>(qb, vb);
+                    // Entry ja of list a: already seen if list b has walked it.
+                    const bool seenA = lex(dbA, ja) <= lastB;
+                    lastA = lex(daA, ja);
+                    const float zA = (seenA || ja == i) ? __builtin_inff() : fmaxf(daA, dbA);
+                    insert_flat<K>(h, id, zA, ja);
+                    // Entry jb of list b: already seen if list a has walked
+                    // it (including ja just now).
+                    const bool seenB = lex(daB, jb) <= lastA;
+                    lastB = lex(dbB, jb);
+                    const float zB = (seenB || jb == i) ? __builtin_inff() : fmaxf(daB, dbB);
+                    insert_flat<K>(h, id, zB, jb);
+                    if (fmaxf(lex_key(lastA), lex_key(lastB)) > h[K] || pos >= n) {
+                        state = 0;
+                        res = true;
                     }
                 }
-                float ua[D], ub[D], va[D], vb[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    ua[d] = ca[d * n + ja];
-                    ub[d] = cb[d * n + ja];
-                    va[d] = ca[d * n + jb];
-                    vb[d] = cb[d * n + jb];
-                }
-                const float daA = marg<D>(qa, ua), dbA = marg<D>(qb, ub);
-                const float daB = marg<D>(qa, va), dbB = marg<D>(qb, vb);
-                // Entry ja of list a: already seen if list b has walked it.
-                const bool seenA = dbA < lb || (dbA == lb && ja <= ljb);
-                la = daA;
-                lja = ja;
-                const float zA = fmaxf(daA, dbA);
-                if (!seenA && ja != i && zA < h[K]) heap_insert<K>(h, id, zA, ja);
-                // Entry jb of list b: already seen if list a has walked it
-                // (including ja just now).
-                const bool seenB = daB < la || (daB == la && jb <= lja);
-                lb = dbB;
-                ljb = jb;
-                const float zB = fmaxf(daB, dbB);
-                if (!seenB && jb != i && zB < h[K]) heap_insert<K>(h, id, zB, jb);
-                if (fmaxf(la, lb) > h[K] || pos >= n) {
-                    state = 0;
-                    res = true;
-                }
+            }
+            if (state == 1) {
+                curA = nxtA;
+                curB = nxtB;
+                nxtA = tA;
+                nxtB = tB;
             }
         }
+        if (pend) cnt[pend_i] = (pend_x - 1u) | ((pend_y - 1u) << 16);
         __syncthreads();  // every count of this pair is written
         if (wib == 0) {
             // psi terms summed sequentially in frame order
