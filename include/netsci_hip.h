@@ -113,7 +113,7 @@ int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symme
  * whenever it applies (at most 2048 frames, finite coordinates, its tables
  * of 4*n*n bytes per used node fit in free device memory) and the all-frames
  * kernel otherwise; 1 = always the all-frames kernel; 2 = the list path or
- * an error. walk_waves (1..8) and walk_batch (1..64) shape the list path's
+ * an error. walk_waves (1..16) and walk_batch (1..64) shape the list path's
  * walk kernel (0 = default). Results are identical for every choice. No
  * reference counterpart (the reference always compares all frame pairs,
  * mutual_information_gpu.cu:62-687). */

@@ -280,7 +280,7 @@ int autotune(const Problem &pb, const Buffers &b, hipStream_t stream, Config &be
 // Method: 0 picks the list path whenever it applies, 1 always runs the
 // all-frames kernel, 2 requires the list path.
 int g_method = 0;
-int g_walk_wpb = 4;    // waves per walk workgroup
+int g_walk_wpb = 16;   // waves per walk workgroup (one workgroup per CU)
 int g_walk_batch = 16; // finished lanes a wave waits for before a refill round
 thread_local long long g_last_ties = 0;
 thread_local int g_last_method = 0;         // 1 all-frames, 2 lists
@@ -300,6 +300,15 @@ ksgw::WalkFn walk_for(int d, int k) {
         case 1: return ksgw::walk_d1(k);
         case 2: return ksgw::walk_d2(k);
         case 3: return ksgw::walk_d3(k);
+        default: return nullptr;
+    }
+}
+
+ksgw::TieFn tie_for(int d, int k) {
+    switch (d) {
+        case 1: return ksgw::tie_d1(k);
+        case 2: return ksgw::tie_d2(k);
+        case 3: return ksgw::tie_d3(k);
         default: return nullptr;
     }
 }
@@ -364,6 +373,26 @@ __global__ __launch_bounds__(1024) void slot_scan_kernel(const int *used, int N,
     if (threadIdx.x == 0) *count = carry;
 }
 
+// psi terms of every frame summed sequentially in frame order, then the MI
+// (mutual_information_cpu.cpp:110-118); one lane per pair.
+__global__ __launch_bounds__(256) void psi_sum_kernel(ksgw::SumArgs a) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.P) return;
+    const unsigned *c = a.cnt + p * a.n;
+    float acc = 0.0f;
+    for (int f = 0; f < a.n; ++f) {
+        const unsigned v = c[f];
+        const int nx = (int)(v & 0xffffu), ny = (int)(v >> 16);
+        acc += a.psi[nx] + a.psi[ny];
+        if (a.counts_out) {
+            a.counts_out[(p * a.n + f) * 2 + 0] = nx;
+            a.counts_out[(p * a.n + f) * 2 + 1] = ny;
+        }
+    }
+    const float avg = acc / (float)a.n;
+    a.mi_out[p] = a.psi[a.n] + a.psi[a.k] - a.inv_k - avg;
+}
+
 // Runs the whole problem on the list path when it applies (one window of at
 // most ksgw::kMaxFrames frames, tables within the free device memory, finite
 // coordinates); otherwise returns kOk with *ran == false.
@@ -378,14 +407,17 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
         return kOk;
     }
     const ksgw::WalkFn walk = walk_for(pb.d, pb.k);
+    const ksgw::TieFn tiek = tie_for(pb.d, pb.k);
     const int width = ksgw::build_width(n);
     const ksgw::BuildFn buildk = build_for(pb.d, width);
-    NSX_REQUIRE(walk && buildk, "ksg: no list-path kernel for (d, k)");
+    NSX_REQUIRE(walk && tiek && buildk, "ksg: no list-path kernel for (d, k)");
     const float *X = pb.X + off;
     const long long m2 = 2 * pb.P;
 
     // Used nodes and their table slots.
-    int *mm = (int *)workspace(kWsListStats, 64);
+    // mm: [0] min node, [1] max node, [4] non-finite flag, [8..11] walk
+    // stats (2 x u64), [16] tie-list count.
+    int *mm = (int *)workspace(kWsListStats, 256);
     if (!mm) return kHipError;
     int rc = fill_u32(mm, 0x7fffffffu, 1, stream);
     if (!rc) rc = fill_u32(mm + 1, 0xffffffffu, 1, stream);
@@ -463,46 +495,85 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
         return kOk;
     }
 
-    // Walk.
-    const int wpb = g_walk_wpb;
-    const size_t wlds = ksgw::walk_lds_bytes(pb.d, n);
+    // Walk: one workgroup of g_walk_wpb waves per CU, as many pair buffers
+    // (up to 3) as fit in its LDS.
+    const int wpb = std::min(g_walk_wpb, ksgw::walk_max_waves(pb.k));
+    const size_t rec_bytes = (size_t)2 * pb.d * n * sizeof(float);
+    const size_t lds_cap = 160 * 1024;
+    int nbuf = (int)std::min<size_t>(3, (lds_cap - 256) / rec_bytes);
+    NSX_REQUIRE(nbuf >= 1, "ksg: the list path's pair records exceed LDS");
+    const size_t wlds = ksgw::walk_lds_bytes(pb.d, n, nbuf);
     int per_cu = 0;
     NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)walk,
                                                          ksgw::kWave * wpb, wlds),
             "walk occupancy");
     per_cu = std::max(per_cu, 1);
-    const long long grid = std::max<long long>(
-        1, std::min<long long>(pb.P, (long long)cu_count() * per_cu));
-    unsigned *cnt = (unsigned *)workspace(kWsListCnt, (size_t)grid * n * sizeof(unsigned));
-    KeyIdx *scratch =
-        (KeyIdx *)workspace(kWsTieScratch, (size_t)grid * wpb * n * sizeof(KeyIdx));
+    const long long grid_max = (long long)cu_count() * per_cu;
+    // Packed counts of one launch: at most 1 GiB (larger problems launch in
+    // pair chunks).
+    const long long chunk = std::max<long long>(
+        1, std::min<long long>(pb.P, (1LL << 30) / ((long long)n * (long long)sizeof(unsigned))));
+    // Tie list: up to 1/32 of a chunk's queries (ties beyond it are
+    // resolved inside the walk).
+    const unsigned tie_cap =
+        (unsigned)std::max<long long>(1024, chunk * (long long)n / 32);
+    const long long tie_waves = (long long)cu_count() * 16;
+    unsigned *cnt = (unsigned *)workspace(
+        kWsListCnt, ((size_t)chunk * n + tie_cap) * sizeof(unsigned));
+    KeyIdx *scratch = (KeyIdx *)workspace(
+        kWsTieScratch,
+        (size_t)std::max<long long>(grid_max * wpb, tie_waves) * n * sizeof(KeyIdx));
     unsigned long long *stats = (unsigned long long *)(mm + 8);
+    unsigned *tie_count = (unsigned *)(mm + 16);
     if (!cnt || !scratch) return kHipError;
+    unsigned *tie_list = cnt + (size_t)chunk * n;
     rc = fill_u32(stats, 0u, 4, stream);
     if (rc) return rc;
-    ksgw::WalkArgs wa;
-    wa.X = X;
-    wa.ld = pb.ld;
-    wa.dstride = pb.dstride;
-    wa.ab = pb.ab;
-    wa.P = pb.P;
-    wa.n = n;
-    wa.k = pb.k;
-    wa.psi = d_psi;
-    wa.inv_k = (float)(1.0 / (float)pb.k);
-    wa.mi_out = mi_out;
-    wa.counts_out = counts_out;
-    wa.ord = ord;
-    wa.rank = rank;
-    wa.rs = rs;
-    wa.slot_of = slot_of;
-    wa.scratch = scratch;
-    wa.cnt = cnt;
-    wa.stats = stats;
-    wa.wpb = wpb;
-    wa.batch = g_walk_batch;
-    hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream, wa);
-    NSX_HIP(hipGetLastError(), "walk launch");
+    for (long long p0 = 0; p0 < pb.P; p0 += chunk) {
+        const long long Pc = std::min(chunk, pb.P - p0);
+        ksgw::WalkArgs wa;
+        wa.X = X;
+        wa.ld = pb.ld;
+        wa.dstride = pb.dstride;
+        wa.ab = pb.ab;
+        wa.p0 = p0;
+        wa.P = Pc;
+        wa.n = n;
+        wa.k = pb.k;
+        wa.ord = ord;
+        wa.rank = rank;
+        wa.rs = rs;
+        wa.slot_of = slot_of;
+        wa.scratch = scratch;
+        wa.cnt = cnt;
+        wa.stats = stats;
+        wa.tie_list = tie_list;
+        wa.tie_count = tie_count;
+        wa.tie_cap = tie_cap;
+        wa.nbuf = nbuf;
+        wa.batch = g_walk_batch;
+        wa.checkpoint = 8;
+        const long long grid = std::max<long long>(1, std::min<long long>(Pc, grid_max));
+        rc = fill_u32(tie_count, 0u, 1, stream);
+        if (rc) return rc;
+        hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream, wa);
+        NSX_HIP(hipGetLastError(), "walk launch");
+        hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / 4)), dim3(256), 0, stream, wa,
+                           scratch);
+        NSX_HIP(hipGetLastError(), "tie launch");
+        ksgw::SumArgs sa;
+        sa.cnt = cnt;
+        sa.P = Pc;
+        sa.n = n;
+        sa.k = pb.k;
+        sa.psi = d_psi;
+        sa.inv_k = (float)(1.0 / (float)pb.k);
+        sa.mi_out = mi_out + p0;
+        sa.counts_out = counts_out ? counts_out + p0 * n * 2 : nullptr;
+        hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)((Pc + 255) / 256)), dim3(256), 0,
+                           stream, sa);
+        NSX_HIP(hipGetLastError(), "psi sum launch");
+    }
     unsigned long long h_stats[2] = {0, 0};
     NSX_HIP(hipMemcpyAsync(h_stats, stats, sizeof(h_stats), hipMemcpyDeviceToHost, stream),
             "walk stats readback");
@@ -634,11 +705,11 @@ int nsx_ksg_set_method(int method, int walk_waves, int walk_batch) {
     clear_error();
     NSX_REQUIRE(method >= 0 && method <= 2, "nsx_ksg_set_method: method must be 0, 1 or 2");
     NSX_REQUIRE(walk_waves >= 0 && walk_waves <= ksgw::kMaxWalkWaves,
-                "nsx_ksg_set_method: walk_waves must be 0..8");
+                "nsx_ksg_set_method: walk_waves must be 0..16");
     NSX_REQUIRE(walk_batch >= 0 && walk_batch <= 64, "nsx_ksg_set_method: walk_batch must be 0..64");
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_method = method;
-    g_walk_wpb = walk_waves ? walk_waves : 4;
+    g_walk_wpb = walk_waves ? walk_waves : 16;
     g_walk_batch = walk_batch ? walk_batch : 16;
     return kOk;
 }

@@ -10,11 +10,13 @@
 //     sorted frame list ord_m[i][.] — together with the upper-bound rank
 //     rank_m[i][j] = #{j' : D_m(i, j') <= D_m(i, j)};
 //   * the joint distance is z(i, j) = max(D_a(i, j), D_b(i, j)), so walking
-//     ord_a[i] and ord_b[i] in step (one entry of each per step) and keeping
-//     the k+1 smallest z of the distinct frames seen so far, every frame not
-//     yet seen in either list has z >= max(last D_a walked, last D_b walked).
-//     Once that bound exceeds the current (k+1)-th smallest z, the k+1
-//     smallest are final (the threshold-algorithm stopping rule).
+//     ord_a[i] and ord_b[i] (in any interleaving) and keeping the k+1
+//     smallest z of the distinct frames seen so far, every frame not yet seen
+//     in either list has z >= max(last D_a walked, last D_b walked). Once that
+//     bound exceeds the current (k+1)-th smallest z, the k+1 smallest are
+//     final (the threshold-algorithm stopping rule). The k+1 nearest lie
+//     within both lists' walked prefixes, so walking mostly the list whose
+//     keys grow fastest is enough.
 //
 // The marginal counts then need no second pass: with eps_x = D_a(i, j*) for
 // the selected frame j* of largest D_a, n_x + 1 = #{j : D_a(i, j) <= eps_x}
@@ -63,7 +65,11 @@ namespace ksgw {
 constexpr int kWave = 64;
 constexpr int kMaxFrames = 2048;  // 32 sort keys per lane
 constexpr int kBuildWaves = 4;
-constexpr int kMaxWalkWaves = 8;
+constexpr int kMaxWalkWaves = 16;
+
+// Waves per walk workgroup a k allows: the heap of k+1 (distance, frame)
+// registers fits 128 VGPRs (16 waves, 4 per SIMD) up to k = 6.
+constexpr int walk_max_waves(int k) { return k <= 6 ? 16 : 8; }
 
 struct BuildArgs {
     const float *X;
@@ -82,30 +88,45 @@ struct WalkArgs {
     const float *X;
     long long ld, dstride;
     const int *ab;
+    long long p0;  // first pair of this launch
+    long long P;   // pairs of this launch
+    int n, k;
+    const uint16_t *ord, *rank;
+    long long rs;
+    const int *slot_of;         // node -> table slot
+    KeyIdx *scratch;            // per wave: n records (tie rows)
+    unsigned *cnt;              // [P][n] packed counts n_x | n_y << 16
+    unsigned long long *stats;  // [0] tie frames, [1] list entries walked (both lists)
+    unsigned *tie_list;         // tied queries (local pair * n + frame), resolved later
+    unsigned *tie_count;        // entries appended (may exceed tie_cap)
+    unsigned tie_cap;
+    int nbuf;                   // pair buffers per workgroup (LDS ring)
+    int batch;                  // lanes that wait before a refill round
+    int checkpoint;             // loop passes between workgroup checkpoints
+};
+
+// Frame-order psi sums of the packed counts (one lane per pair).
+struct SumArgs {
+    const unsigned *cnt;
     long long P;
     int n, k;
     const float *psi;
     float inv_k;
-    float *mi_out;
-    int *counts_out;  // nullable: [p][frame][2]
-    const uint16_t *ord, *rank;
-    long long rs;
-    const int *slot_of;       // node -> table slot
-    KeyIdx *scratch;          // per wave: n records (tie rows)
-    unsigned *cnt;            // per workgroup: n packed counts n_x | n_y << 16
-    unsigned long long *stats;  // [0] tie frames, [1] list steps (entries per list)
-    int wpb;
-    int batch;
+    float *mi_out;    // [P]
+    int *counts_out;  // nullable: [P][n][2]
 };
 
 using BuildFn = void (*)(BuildArgs);
 using WalkFn = void (*)(WalkArgs);
+using TieFn = void (*)(WalkArgs, KeyIdx *);
 
 // LDS bytes of the two kernels.
 inline size_t build_lds_bytes(int d, int n, long long rs) {
     return (((size_t)d * n * 4 + 15) & ~(size_t)15) + (size_t)kBuildWaves * rs * 2;
 }
-inline size_t walk_lds_bytes(int d, int n) { return (size_t)2 * d * n * 4 + 16; }
+inline size_t walk_lds_bytes(int d, int n, int nbuf) {
+    return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 4 * nbuf) * sizeof(int);
+}
 
 // Sort keys per lane for n frames (8, 16 or 32).
 inline int build_width(int n) { return n <= 512 ? 8 : n <= 1024 ? 16 : 32; }
@@ -117,6 +138,9 @@ BuildFn build_d3(int width);
 WalkFn walk_d1(int k);
 WalkFn walk_d2(int k);
 WalkFn walk_d3(int k);
+TieFn tie_d1(int k);
+TieFn tie_d2(int k);
+TieFn tie_d3(int k);
 
 #if defined(__HIPCC__)
 
@@ -309,272 +333,460 @@ __device__ __forceinline__ int block_entry(const uint4 &b, int q) {
     return (int)((w >> ((q & 1) * 16)) & 0xffffu);
 }
 
+// Stage local pair s of this workgroup into ring buffer s % nbuf: both rows'
+// coordinates as per-frame records, and the two nodes' table slots.
+template <int D>
+__device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *meta, long long s,
+                                           int n) {
+    const long long p = a.p0 + (long long)blockIdx.x + s * (long long)gridDim.x;
+    const int b = (int)(s % a.nbuf);
+    const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
+    const float *xa = a.X + (long long)na * a.ld;
+    const float *xb = a.X + (long long)nb * a.ld;
+    float *rec = rec0 + (long long)b * 2 * D * n;
+    for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
+        const int d = t / n, f = t - d * n;
+        rec[2 * D * f + d] = xa[d * a.dstride + f];
+        rec[2 * D * f + D + d] = xb[d * a.dstride + f];
+    }
+    if (threadIdx.x == 0) {
+        meta[2 * b] = a.slot_of[na];
+        meta[2 * b + 1] = a.slot_of[nb];
+    }
+}
+
+// One persistent workgroup per CU walks the queries of its pairs
+// blockIdx.x, blockIdx.x + gridDim.x, ... (local pairs 0, 1, ...). Query q
+// is frame q % n of local pair q / n; lanes reserve query numbers from one
+// counter, so when a pair's queries run out its lanes go on to the next pair
+// while the pair's longest walks finish. Up to nbuf pairs are staged at once
+// (an LDS ring); every `checkpoint` loop passes all waves meet, the pairs
+// whose queries are all done retire in order and free their buffers for the
+// next pairs.
 template <int D, int K>
-__global__ __launch_bounds__(kWave *kMaxWalkWaves) void walk_kernel(WalkArgs a) {
+__global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int n = a.n;
-    const float *rec = lds;  // [n][2D]: frame j's node-a then node-b coordinates
-    int *counter = reinterpret_cast<int *>(lds + 2 * D * n);
+    const int nbuf = a.nbuf;
+    const int rec_floats = 2 * D * n;
+    int *ctrl = reinterpret_cast<int *>(lds + nbuf * rec_floats);
+    // ctrl[0] next query, [1] staged pairs, [2] retired pairs, [3] pairs to
+    // stage now, [4..4+nbuf) done queries per buffer, then 2 table slots per
+    // buffer, then the pairs to stage.
+    int *done = ctrl + 8;
+    int *meta = done + nbuf;
+    int *to_stage = meta + 2 * nbuf;
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    KeyIdx *scratch = a.scratch + ((long long)blockIdx.x * a.wpb + wib) * n;
-    unsigned *cnt = a.cnt + (long long)blockIdx.x * n;
+    const int wpb = blockDim.x / kWave;
+    KeyIdx *scratch = a.scratch + ((long long)blockIdx.x * wpb + wib) * n;
     const long long rs = a.rs;
-    const long long table = (long long)n * rs;
+    const long long G = gridDim.x;
+    const long long npairs =
+        (long long)blockIdx.x < a.P ? (a.P - (long long)blockIdx.x + G - 1) / G : 0;
+    const long long nq = npairs * n;
     unsigned long long ties = 0, steps = 0;
 
-    for (long long p = blockIdx.x; p < a.P; p += gridDim.x) {
-        const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
-        const float *xa = a.X + (long long)na * a.ld;
-        const float *xb = a.X + (long long)nb * a.ld;
-        const uint16_t *ordA = a.ord + (long long)a.slot_of[na] * table;
-        const uint16_t *ordB = a.ord + (long long)a.slot_of[nb] * table;
-        const uint16_t *rankA = a.rank + (long long)a.slot_of[na] * table;
-        const uint16_t *rankB = a.rank + (long long)a.slot_of[nb] * table;
-        __syncthreads();  // the previous pair's readers are done
-        for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
-            const int d = t / n, f = t - d * n;
-            lds[2 * D * f + d] = xa[d * a.dstride + f];
-            lds[2 * D * f + D + d] = xb[d * a.dstride + f];
-        }
-        if (threadIdx.x == 0) *counter = 0;
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        ctrl[0] = ctrl[1] = ctrl[2] = ctrl[3] = 0;
+        for (int b = 0; b < nbuf; ++b) done[b] = 0;
+    }
+    const int first = (int)(npairs < nbuf ? npairs : nbuf);
+    for (int s = 0; s < first; ++s) stage_pair<D>(a, lds, meta, s, n);
+    __syncthreads();
+    if (threadIdx.x == 0) ctrl[1] = first;
 
-        // Lane state: 0 wants a query, 1 walks, 2 no queries left.
-        int state = 0;
-        bool res = false;  // a finished walk waits for its counts
-        int i = 0, pos = 0;
-        // Last walked entry of each list as lex(D, j); -1: none yet (its key
-        // reads as NaN, so the stop test below is false).
-        long long lastA = -1, lastB = -1;
-        float qa[D], qb[D], h[K + 1];
-        int id[K + 1];
-        // List blocks: cur holds entries [pos, pos + 8) of each list, nxt
-        // the 8 after them (pos is a multiple of 8 at every body start).
-        uint4 curA = {0, 0, 0, 0}, curB = {0, 0, 0, 0}, nxtA = {0, 0, 0, 0},
-              nxtB = {0, 0, 0, 0};
-        const uint16_t *rowA = ordA, *rowB = ordB;
-        // Counts of the previous refill round, stored one round later so
-        // that the rank lookups' latency is not waited for.
-        bool pend = false;
-        int pend_i = 0;
-        unsigned pend_x = 1u, pend_y = 1u;
+    // Lane state: 0 wants a query, 3 holds a reserved query whose pair is
+    // not staged yet, 1 walks, 2 no queries left.
+    int state = 0;
+    bool res = false;  // a finished walk waits for its counts
+    long long q = 0;   // the lane's query (local pair q / n, frame q % n)
+    int i = 0, posA = 0, posB = 0;  // entries walked in each list
+    int rec = 0;  // LDS float offset of the lane's pair records (an offset, not a
+                  // pointer, so every access stays an LDS instruction)
+    long long offA = 0, offB = 0;  // table row offsets of (node, frame i)
+    // Last walked entry of each list as lex(D, j); -1: none yet (its key
+    // reads as NaN, so the stop test below is false).
+    long long lastA = -1, lastB = -1;
+    float qa[D], qb[D], h[K + 1];
+    int id[K + 1];
+    // List blocks: cur holds entries [pos, pos + 8) of each list, nxt the 8
+    // after them (each list's pos is a multiple of 8 at every body start,
+    // until the list is exhausted).
+    uint4 curA = {0, 0, 0, 0}, curB = {0, 0, 0, 0}, nxtA = {0, 0, 0, 0}, nxtB = {0, 0, 0, 0};
+    // Counts of the previous refill round, stored one round later so that
+    // the rank lookups' latency is not waited for.
+    bool pend = false;
+    long long pend_at = 0;
+    unsigned pend_x = 1u, pend_y = 1u;
 #pragma unroll
-        for (int d = 0; d < D; ++d) qa[d] = qb[d] = 0.0f;
+    for (int d = 0; d < D; ++d) qa[d] = qb[d] = 0.0f;
 #pragma unroll
-        for (int m = 0; m <= K; ++m) {
-            h[m] = __builtin_inff();
-            id[m] = 0;
-        }
+    for (int m = 0; m <= K; ++m) {
+        h[m] = __builtin_inff();
+        id[m] = 0;
+    }
 
-        while (true) {
-            const unsigned long long want = __ballot(state == 0);
-            const unsigned long long walking = __ballot(state == 1);
-            if (want && (__popcll(want) >= (unsigned)a.batch || walking == 0)) {
-                // (0) counts looked up in the previous round.
-                if (pend) cnt[pend_i] = (pend_x - 1u) | ((pend_y - 1u) << 16);
-                pend = false;
-                // (1) eps of the finished walks.
-                const bool tie = res && (h[K - 1] == h[K]);
-                int jx = 0, jy = 0;
-                if (res && !tie) {
+    for (int pass = 0;; ++pass) {
+        if (pass % a.checkpoint == 0) {
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int staged = ctrl[1], retired = ctrl[2], ns = 0;
+                while (retired < staged && done[retired % nbuf] == n) {
+                    done[retired % nbuf] = 0;
+                    ++retired;
+                    if (staged + ns < npairs) {
+                        to_stage[ns] = staged + ns;
+                        ++ns;
+                    }
+                }
+                ctrl[2] = retired;
+                ctrl[3] = ns;
+            }
+            __syncthreads();
+            const int ns = ctrl[3];
+            if (ns > 0) {
+                for (int x = 0; x < ns; ++x) stage_pair<D>(a, lds, meta, to_stage[x], n);
+                __syncthreads();
+                if (threadIdx.x == 0) ctrl[1] += ns;
+                __syncthreads();
+            }
+            if (ctrl[2] >= npairs) break;
+        }
+        const long long avail = (long long)ctrl[1] * n;  // queries of staged pairs
+        const unsigned long long want = __ballot(state == 0);
+        const unsigned long long ready = __ballot(state == 3 && q < avail);
+        const unsigned long long walking = __ballot(state == 1);
+        if ((want | ready) &&
+            (__popcll(want) + __popcll(ready) >= (unsigned)a.batch || walking == 0)) {
+            // (0) counts looked up in the previous round.
+            if (pend) a.cnt[pend_at] = (pend_x - 1u) | ((pend_y - 1u) << 16);
+            pend = false;
+            // (1) eps of the finished walks.
+            const bool tie = res && (h[K - 1] == h[K]);
+            int jx = 0, jy = 0;
+            if (res && !tie) {
+                float ex = -1.0f, ey = -1.0f;
+#pragma unroll
+                for (int m = 0; m < K; ++m) {
+                    float va[D], vb[D];
+                    fetch<D>(lds + rec, id[m], va, vb);
+                    const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+                    jx = da > ex ? id[m] : jx;
+                    ex = fmaxf(ex, da);
+                    jy = db > ey ? id[m] : jy;
+                    ey = fmaxf(ey, db);
+                }
+            }
+            // Ties go to the tie list (resolved by tie_kernel after the walk);
+            // only when it is full are they resolved here.
+            unsigned long long tm = __ballot(tie);
+            ties += (unsigned)__popcll(tm);
+            bool deferred = false;
+            if (tm) {
+                const int leader = __builtin_amdgcn_readfirstlane(__ffsll((long long)tm) - 1);
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(a.tie_count, (unsigned)__popcll(tm));
+                base = __shfl(base, leader);
+                const unsigned slot = base + __builtin_amdgcn_mbcnt_hi(
+                                                 (unsigned)(tm >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned)tm, 0u));
+                if (tie && slot < a.tie_cap) {
+                    const long long s = q / n;
+                    a.tie_list[slot] = (unsigned)(((long long)blockIdx.x + s * G) * n + i);
+                    deferred = true;
+                }
+                tm = __ballot(tie && !deferred);
+            }
+            while (tm) {
+                // The reference order: std::sort of the full joint row, eps
+                // over positions 1..k (mutual_information_cpu.cpp:48-77).
+                const int t = __builtin_amdgcn_readfirstlane(__ffsll((long long)tm) - 1);
+                float tqa[D], tqb[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    tqa[d] = __shfl(qa[d], t);
+                    tqb[d] = __shfl(qb[d], t);
+                }
+                const int trec = __shfl(rec, t);
+                for (int j = lane; j < n; j += kWave) {
+                    float va[D], vb[D];
+                    fetch<D>(lds + trec, j, va, vb);
+                    KeyIdx r;
+                    r.key = fmaxf(marg<D>(tqa, va), marg<D>(tqb, vb));
+                    r.idx = j;
+                    scratch[j] = r;
+                }
+                ksg::wave_sync_global();
+                int rjx = 0, rjy = 0;
+                if (lane == 0) {
+                    sort_prefix(scratch, n, K + 1);
                     float ex = -1.0f, ey = -1.0f;
-#pragma unroll
-                    for (int m = 0; m < K; ++m) {
+                    for (int m = 1; m <= K; ++m) {
+                        const int j = scratch[m].idx;
                         float va[D], vb[D];
-                        fetch<D>(rec, id[m], va, vb);
-                        const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
-                        jx = da > ex ? id[m] : jx;
+                        fetch<D>(lds + trec, j, va, vb);
+                        const float da = marg<D>(tqa, va), db = marg<D>(tqb, vb);
+                        rjx = da > ex ? j : rjx;
                         ex = fmaxf(ex, da);
-                        jy = db > ey ? id[m] : jy;
+                        rjy = db > ey ? j : rjy;
                         ey = fmaxf(ey, db);
                     }
                 }
-                unsigned long long tm = __ballot(tie);
-                ties += (unsigned)__popcll(tm);
-                while (tm) {
-                    // The reference order: std::sort of the full joint row,
-                    // eps over positions 1..k (mutual_information_cpu.cpp:48-77).
-                    const int t = __builtin_amdgcn_readfirstlane(__ffsll((long long)tm) - 1);
-                    float tqa[D], tqb[D];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) {
-                        tqa[d] = __shfl(qa[d], t);
-                        tqb[d] = __shfl(qb[d], t);
-                    }
-                    for (int j = lane; j < n; j += kWave) {
-                        float va[D], vb[D];
-                        fetch<D>(rec, j, va, vb);
-                        KeyIdx r;
-                        r.key = fmaxf(marg<D>(tqa, va), marg<D>(tqb, vb));
-                        r.idx = j;
-                        scratch[j] = r;
-                    }
-                    ksg::wave_sync_global();
-                    int rjx = 0, rjy = 0;
-                    if (lane == 0) {
-                        sort_prefix(scratch, n, K + 1);
-                        float ex = -1.0f, ey = -1.0f;
-                        for (int m = 1; m <= K; ++m) {
-                            const int j = scratch[m].idx;
-                            float va[D], vb[D];
-                            fetch<D>(rec, j, va, vb);
-                            const float da = marg<D>(tqa, va), db = marg<D>(tqb, vb);
-                            rjx = da > ex ? j : rjx;
-                            ex = fmaxf(ex, da);
-                            rjy = db > ey ? j : rjy;
-                            ey = fmaxf(ey, db);
-                        }
-                    }
-                    rjx = __shfl(rjx, 0);
-                    rjy = __shfl(rjy, 0);
-                    jx = lane == t ? rjx : jx;
-                    jy = lane == t ? rjy : jy;
-                    ksg::wave_sync_global();
-                    tm &= tm - 1;
-                }
-                if (res) {
-                    pend_x = rankA[(long long)i * rs + jx];
-                    pend_y = rankB[(long long)i * rs + jy];
-                    pend_i = i;
+                rjx = __shfl(rjx, 0);
+                rjy = __shfl(rjy, 0);
+                jx = lane == t ? rjx : jx;
+                jy = lane == t ? rjy : jy;
+                ksg::wave_sync_global();
+                tm &= tm - 1;
+            }
+            if (res) {
+                const long long s = q / n;
+                if (!deferred) {
+                    pend_x = a.rank[offA + jx];
+                    pend_y = a.rank[offB + jy];
+                    pend_at = ((long long)blockIdx.x + s * G) * n + i;
                     pend = true;
-                    steps += (unsigned)pos;
                 }
-                res = false;
-                // (2) new queries for every lane that wants one.
+                steps += (unsigned)(posA + posB);
+                atomicAdd(done + (int)(s % nbuf), 1);  // its LDS records are no longer read
+            }
+            res = false;
+            // (2) reserve query numbers for the lanes that want one.
+            if (want) {
                 const int leader = __builtin_amdgcn_readfirstlane(__ffsll((long long)want) - 1);
                 int base = 0;
-                if (lane == leader) base = atomicAdd(counter, __popcll(want));
+                if (lane == leader) base = atomicAdd(ctrl, __popcll(want));
                 base = __shfl(base, leader);
                 const int below = __builtin_amdgcn_mbcnt_hi(
                     (unsigned)(want >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)want, 0u));
                 if (state == 0) {
-                    const int qi = base + below;
-                    if (qi < n) {
-                        i = qi;
-                        fetch<D>(rec, i, qa, qb);
-#pragma unroll
-                        for (int m = 0; m <= K; ++m) {
-                            h[m] = __builtin_inff();
-                            id[m] = 0;
-                        }
-                        pos = 0;
-                        lastA = lastB = -1;
-                        rowA = ordA + (long long)i * rs;
-                        rowB = ordB + (long long)i * rs;
-                        curA = load_block(rowA);
-                        curB = load_block(rowB);
-                        if (rs > 8) {
-                            nxtA = load_block(rowA + 8);
-                            nxtB = load_block(rowB + 8);
-                        }
-                        state = 1;
-                    } else {
-                        state = 2;
-                    }
+                    q = base + below;
+                    state = q < nq ? 3 : 2;
                 }
             }
-            if (__ballot(state == 1) == 0) break;
-            // A body: 8 steps, one entry of each list per step. The block
-            // two ahead is requested first and used one body later.
-            uint4 tA = nxtA, tB = nxtB;
-            if (state == 1 && pos + 16 < rs) {
-                tA = load_block(rowA + pos + 16);
-                tB = load_block(rowB + pos + 16);
-            }
+            // (3) start the reserved queries whose pairs are staged.
+            if (state == 3 && q < avail) {
+                const long long s = q / n;
+                const int b = (int)(s % nbuf);
+                i = (int)(q - s * n);
+                rec = b * rec_floats;
+                fetch<D>(lds + rec, i, qa, qb);
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                if (state == 1) {
-                    const int ja = block_entry(curA, s), jb = block_entry(curB, s);
-                    ++pos;
-                    float ua[D], ub[D], va[D], vb[D];
-                    fetch<D>(rec, ja, ua, ub);
-                    fetch<D>(rec, jb, va, vb);
-                    const float daA = marg<D>(qa, ua), dbA = marg<D>(qb, ub);
-                    const float daB = marg<D>(qa, va), dbB = marg<D>(qb, vb);
-                    // Entry ja of list a: already seen if list b has walked it.
-                    const bool seenA = lex(dbA, ja) <= lastB;
-                    lastA = lex(daA, ja);
-                    const float zA = (seenA || ja == i) ? __builtin_inff() : fmaxf(daA, dbA);
-                    insert_flat<K>(h, id, zA, ja);
-                    // Entry jb of list b: already seen if list a has walked
-                    // it (including ja just now).
-                    const bool seenB = lex(daB, jb) <= lastA;
-                    lastB = lex(dbB, jb);
-                    const float zB = (seenB || jb == i) ? __builtin_inff() : fmaxf(daB, dbB);
-                    insert_flat<K>(h, id, zB, jb);
-                    if (fmaxf(lex_key(lastA), lex_key(lastB)) > h[K] || pos >= n) {
+                for (int m = 0; m <= K; ++m) {
+                    h[m] = __builtin_inff();
+                    id[m] = 0;
+                }
+                posA = posB = 0;
+                lastA = lastB = -1;
+                offA = ((long long)meta[2 * b] * n + i) * rs;
+                offB = ((long long)meta[2 * b + 1] * n + i) * rs;
+                curA = load_block(a.ord + offA);
+                curB = load_block(a.ord + offB);
+                if (rs > 8) {
+                    nxtA = load_block(a.ord + offA + 8);
+                    nxtB = load_block(a.ord + offB + 8);
+                }
+                state = 1;
+            }
+        }
+        if (__ballot(state == 1) == 0) {
+            pass = (pass / a.checkpoint + 1) * a.checkpoint - 1;  // on to the checkpoint
+            continue;
+        }
+        // A body: the next 8 entries of one list. The lane takes the list
+        // whose walk is expected to end sooner: after the first block of each,
+        // a list's remaining entries below h[k] are estimated as
+        // pos * ((h[k] / last key)^d - 1) (a d-dimensional ball holds about
+        // r^d of the frames). Any choice gives the same result; this one
+        // walks about 40% fewer entries than taking one of each per step.
+        bool useA;
+        {
+            const float hk = h[K];
+            if (posA >= n) {
+                useA = false;
+            } else if (posB >= n) {
+                useA = true;
+            } else if (posA < 8 || posB < 8 || !(hk < __builtin_inff())) {
+                useA = posA <= posB;
+            } else {
+                const float ra = hk / fmaxf(lex_key(lastA), 1e-30f);
+                const float rb = hk / fmaxf(lex_key(lastB), 1e-30f);
+                float pa = ra, pb = rb;
+#pragma unroll
+                for (int d = 1; d < D; ++d) {
+                    pa *= ra;
+                    pb *= rb;
+                }
+                useA = (float)posA * (pa - 1.0f) <= (float)posB * (pb - 1.0f);
+            }
+        }
+        const uint4 blk = useA ? curA : curB;
+        const long long off = useA ? offA : offB;
+        const long long lastO = useA ? lastB : lastA;  // fixed during the body
+        const int posO = useA ? posB : posA;
+        long long lastL = useA ? lastA : lastB;
+        int posL = useA ? posA : posB;
+        // The chosen list's block two ahead is requested now and used one
+        // body later.
+        uint4 tmp = useA ? nxtA : nxtB;
+        if (state == 1 && posL + 16 < rs) tmp = load_block(a.ord + off + posL + 16);
+        bool on = state == 1;
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            if (on) {
+                const int j = block_entry(blk, st);
+                ++posL;
+                float u[D], v[D];
+                fetch<D>(lds + rec, j, u, v);
+                const float da = marg<D>(qa, u), db = marg<D>(qb, v);
+                const float keyL = useA ? da : db, keyO = useA ? db : da;
+                // Already evaluated if the other list has walked it.
+                const bool seen = lex(keyO, j) <= lastO;
+                lastL = lex(keyL, j);
+                const float z = (seen || j == i) ? __builtin_inff() : fmaxf(da, db);
+                insert_flat<K>(h, id, z, j);
+                if (fmaxf(lex_key(lastL), lex_key(lastO)) > h[K]) {
+                    state = 0;
+                    res = true;
+                    on = false;
+                } else if (posL >= n) {
+                    on = false;  // this list is exhausted
+                    if (posO >= n) {
                         state = 0;
                         res = true;
                     }
                 }
             }
-            if (state == 1) {
-                curA = nxtA;
-                curB = nxtB;
-                nxtA = tA;
-                nxtB = tB;
-            }
         }
-        if (pend) cnt[pend_i] = (pend_x - 1u) | ((pend_y - 1u) << 16);
-        __syncthreads();  // every count of this pair is written
-        if (wib == 0) {
-            // psi terms summed sequentially in frame order
-            // (mutual_information_cpu.cpp:110-118).
-            float acc = 0.0f;
-            for (int f0 = 0; f0 < n; f0 += kWave) {
-                const int f = f0 + lane;
-                float term = 0.0f;
-                if (f < n) {
-                    const unsigned c = cnt[f];
-                    const int nx = (int)(c & 0xffffu), ny = (int)(c >> 16);
-                    term = a.psi[nx] + a.psi[ny];
-                    if (a.counts_out) {
-                        a.counts_out[(p * n + f) * 2 + 0] = nx;
-                        a.counts_out[(p * n + f) * 2 + 1] = ny;
-                    }
-                }
-                const int lim = min(kWave, n - f0);
-                for (int l = 0; l < lim; ++l)
-                    acc += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(term), l));
+        if (useA) {
+            lastA = lastL;
+            posA = posL;
+        } else {
+            lastB = lastL;
+            posB = posL;
+        }
+        if (state == 1) {
+            if (useA) {
+                curA = nxtA;
+                nxtA = tmp;
+            } else {
+                curB = nxtB;
+                nxtB = tmp;
             }
-            const float avg = acc / (float)n;
-            const float mi = a.psi[n] + a.psi[a.k] - a.inv_k - avg;
-            if (lane == 0) a.mi_out[p] = mi;
         }
     }
+    if (pend) a.cnt[pend_at] = (pend_x - 1u) | ((pend_y - 1u) << 16);
     // Per-wave totals.
-    unsigned long long s = steps;
+    unsigned long long sum = steps;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+    for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off);
     if (lane == 0) {
         if (ties) atomicAdd(a.stats, ties);
-        atomicAdd(a.stats + 1, s);
+        atomicAdd(a.stats + 1, sum);
+    }
+}
+
+// Resolve the tied queries the walk deferred: one wave per entry rebuilds
+// the query's joint distance row, selects positions 0..k in libstdc++
+// std::sort order (mutual_information_cpu.cpp:48-77), takes eps over
+// positions 1..k and stores the counts from the rank tables.
+template <int D, int K>
+__global__ __launch_bounds__(256) void tie_kernel(WalkArgs a, KeyIdx *scratch_all) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
+    const int n = a.n;
+    KeyIdx *scratch = scratch_all + wave * n;
+    const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
+    for (long long e = wave; e < count; e += nwaves) {
+        const unsigned v = a.tie_list[e];
+        const long long pl = v / (unsigned)n;
+        const int i = (int)(v - pl * n);
+        const long long p = a.p0 + pl;
+        const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
+        const float *xa = a.X + (long long)na * a.ld;
+        const float *xb = a.X + (long long)nb * a.ld;
+        float qa[D], qb[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            qa[d] = xa[d * a.dstride + i];
+            qb[d] = xb[d * a.dstride + i];
+        }
+        for (int j = lane; j < n; j += kWave) {
+            float va[D], vb[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                va[d] = xa[d * a.dstride + j];
+                vb[d] = xb[d * a.dstride + j];
+            }
+            KeyIdx r;
+            r.key = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
+            r.idx = j;
+            scratch[j] = r;
+        }
+        ksg::wave_sync_global();
+        if (lane == 0) {
+            sort_prefix(scratch, n, K + 1);
+            float ex = -1.0f, ey = -1.0f;
+            int jx = 0, jy = 0;
+            for (int m = 1; m <= K; ++m) {
+                const int j = scratch[m].idx;
+                float va[D], vb[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    va[d] = xa[d * a.dstride + j];
+                    vb[d] = xb[d * a.dstride + j];
+                }
+                const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+                jx = da > ex ? j : jx;
+                ex = fmaxf(ex, da);
+                jy = db > ey ? j : jy;
+                ey = fmaxf(ey, db);
+            }
+            const long long offA = ((long long)a.slot_of[na] * n + i) * a.rs;
+            const long long offB = ((long long)a.slot_of[nb] * n + i) * a.rs;
+            const unsigned rx = a.rank[offA + jx], ry = a.rank[offB + jy];
+            a.cnt[pl * n + i] = (rx - 1u) | ((ry - 1u) << 16);
+        }
+        ksg::wave_sync_global();
     }
 }
 
 template <int D, int K>
 struct WalkInstances {
-    static void fill(WalkFn *t) {
+    static void fill(WalkFn *t, TieFn *u) {
         t[K] = walk_kernel<D, K>;
-        WalkInstances<D, K - 1>::fill(t);
+        u[K] = tie_kernel<D, K>;
+        WalkInstances<D, K - 1>::fill(t, u);
     }
 };
 template <int D>
 struct WalkInstances<D, 0> {
-    static void fill(WalkFn *) {}
+    static void fill(WalkFn *, TieFn *) {}
 };
 
 template <int D>
 struct WalkTable {
     WalkFn t[ksg::kMaxK + 1] = {};
-    WalkTable() { WalkInstances<D, ksg::kMaxK>::fill(t); }
+    TieFn u[ksg::kMaxK + 1] = {};
+    WalkTable() { WalkInstances<D, ksg::kMaxK>::fill(t, u); }
 };
 
 template <int D>
-WalkFn walk_lookup(int k) {
+const WalkTable<D> &walk_table() {
     static const WalkTable<D> tab;
-    return (k >= 1 && k <= ksg::kMaxK) ? tab.t[k] : nullptr;
+    return tab;
+}
+
+template <int D>
+WalkFn walk_lookup(int k) {
+    return (k >= 1 && k <= ksg::kMaxK) ? walk_table<D>().t[k] : nullptr;
+}
+
+template <int D>
+TieFn tie_lookup(int k) {
+    return (k >= 1 && k <= ksg::kMaxK) ? walk_table<D>().u[k] : nullptr;
 }
 
 template <int D>

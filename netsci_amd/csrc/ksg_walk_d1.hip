@@ -7,6 +7,7 @@ namespace ksgw {
 
 BuildFn build_d1(int width) { return build_lookup<1>(width); }
 WalkFn walk_d1(int k) { return walk_lookup<1>(k); }
+TieFn tie_d1(int k) { return tie_lookup<1>(k); }
 
 }  // namespace ksgw
 }  // namespace nsx
