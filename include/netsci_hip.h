@@ -120,9 +120,15 @@ int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symme
 int nsx_ksg_set_method(int method, int walk_waves, int walk_batch);
 
 /* Which algorithm the last nsx_ksg_mi call on this thread ran (1 all-frames,
- * 2 sorted lists) and, for the list path, the list entries walked (per
- * list, summed over all query frames). */
+ * 2 sorted lists) and, for the list path, the list entries walked (both
+ * lists, summed over all query frames). */
 int nsx_ksg_last_method(int *method, long long *list_entries);
+
+/* Device time (ms, hipEvents on the call's stream) of the last list-path
+ * nsx_ksg_mi call's phases: phase_ms[0] node map and table build, [1] walk,
+ * [2] tie resolution, [3] frame-order psi sums. A diagnostic for roofline
+ * accounting; no reference counterpart. */
+int nsx_ksg_last_phase_ms(double *phase_ms);
 
 /* ---- batched plans: one hipGraph launch per replay ----------------------
  * A plan fixes X, the windows, the pairs and the output of

@@ -41,6 +41,7 @@ SIGNATURES = {
     "nsx_ksg_set_config": [_I, _I, _I, _I, _I],
     "nsx_ksg_set_method": [_I, _I, _I],
     "nsx_ksg_last_method": [_IP, _LLP],
+    "nsx_ksg_last_phase_ms": [ctypes.POINTER(ctypes.c_double)],
     "nsx_ksg_plan_create": [ctypes.POINTER(_VP), _VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I,
                             _VP, _I],
     "nsx_ksg_plan_launch": [_VP, _VP],
@@ -230,10 +231,17 @@ def set_method(method=METHOD_AUTO, walk_waves=0, walk_batch=0):
 
 def last_method():
     """(method the last ksg_mi call ran: 1 all-frames / 2 lists, list entries
-    walked per list)."""
+    walked, both lists)."""
     m, e = ctypes.c_int(0), ctypes.c_longlong(0)
     lib().nsx_ksg_last_method(ctypes.byref(m), ctypes.byref(e))
     return m.value, e.value
+
+
+def last_phase_ms():
+    """Device ms of the last list-path call: {build, walk, ties, psi_sums}."""
+    out = (ctypes.c_double * 4)()
+    lib().nsx_ksg_last_phase_ms(out)
+    return {"build": out[0], "walk": out[1], "ties": out[2], "psi_sums": out[3]}
 
 
 def last_tie_frames():

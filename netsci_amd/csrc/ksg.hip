@@ -284,7 +284,24 @@ int g_walk_wpb = 16;   // waves per walk workgroup (one workgroup per CU)
 int g_walk_batch = 16; // finished lanes a wave waits for before a refill round
 thread_local long long g_last_ties = 0;
 thread_local int g_last_method = 0;         // 1 all-frames, 2 lists
-thread_local long long g_last_entries = 0;  // list entries walked (per list)
+thread_local long long g_last_entries = 0;  // list entries walked (both lists)
+// Device time of the last list-path call's phases (hipEvents on its stream):
+// [0] node map + table build, [1] walk, [2] tie resolution, [3] psi sums.
+thread_local double g_last_phase_ms[4] = {0, 0, 0, 0};
+
+// Event pairs for phase timing, created once per thread.
+struct PhaseEvents {
+    std::vector<hipEvent_t> ev;
+    hipEvent_t get(size_t i) {
+        while (ev.size() <= i) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev.push_back(e);
+        }
+        return ev[i];
+    }
+};
+thread_local PhaseEvents g_phase_events;
 
 ksgw::BuildFn build_for(int d, int width) {
     switch (d) {
@@ -374,23 +391,45 @@ __global__ __launch_bounds__(1024) void slot_scan_kernel(const int *used, int N,
 }
 
 // psi terms of every frame summed sequentially in frame order, then the MI
-// (mutual_information_cpu.cpp:110-118); one lane per pair.
-__global__ __launch_bounds__(256) void psi_sum_kernel(ksgw::SumArgs a) {
-    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.P) return;
-    const unsigned *c = a.cnt + p * a.n;
+// (mutual_information_cpu.cpp:110-118). A wave owns 64 consecutive pairs,
+// one lane each: tiles of 64 pairs x 64 frames are read row by row
+// (coalesced), turned into psi terms and transposed through LDS, and every
+// lane adds its own pair's terms in frame order.
+constexpr int kSumWaves = 4;
+__global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a) {
+    __shared__ float tile[kSumWaves][64][65];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x / 64;
+    const long long p0 = ((long long)blockIdx.x * kSumWaves + w) * 64;
+    if (p0 >= a.P) return;
+    const int rows = (int)min(64LL, a.P - p0);
     float acc = 0.0f;
-    for (int f = 0; f < a.n; ++f) {
-        const unsigned v = c[f];
-        const int nx = (int)(v & 0xffffu), ny = (int)(v >> 16);
-        acc += a.psi[nx] + a.psi[ny];
-        if (a.counts_out) {
-            a.counts_out[(p * a.n + f) * 2 + 0] = nx;
-            a.counts_out[(p * a.n + f) * 2 + 1] = ny;
+    for (int f0 = 0; f0 < a.n; f0 += 64) {
+        const int f = f0 + lane;
+        const int cols = min(64, a.n - f0);
+        for (int r = 0; r < rows; ++r) {
+            float term = 0.0f;
+            if (f < a.n) {
+                const long long at = (p0 + r) * a.n + f;
+                const unsigned v = a.cnt[at];
+                const int nx = (int)(v & 0xffffu), ny = (int)(v >> 16);
+                term = a.psi[nx] + a.psi[ny];
+                if (a.counts_out) {
+                    a.counts_out[at * 2 + 0] = nx;
+                    a.counts_out[at * 2 + 1] = ny;
+                }
+            }
+            tile[w][r][lane] = term;
         }
+        ksg::wave_sync();
+        if (lane < rows)
+            for (int c = 0; c < cols; ++c) acc += tile[w][lane][c];
+        ksg::wave_sync();
     }
-    const float avg = acc / (float)a.n;
-    a.mi_out[p] = a.psi[a.n] + a.psi[a.k] - a.inv_k - avg;
+    if (lane < rows) {
+        const float avg = acc / (float)a.n;
+        a.mi_out[p0 + lane] = a.psi[a.n] + a.psi[a.k] - a.inv_k - avg;
+    }
 }
 
 // Runs the whole problem on the list path when it applies (one window of at
@@ -413,6 +452,9 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     NSX_REQUIRE(walk && tiek && buildk, "ksg: no list-path kernel for (d, k)");
     const float *X = pb.X + off;
     const long long m2 = 2 * pb.P;
+    for (int t = 0; t < 4; ++t) g_last_phase_ms[t] = 0;
+    hipEvent_t ev_start = g_phase_events.get(0);
+    if (ev_start) (void)hipEventRecord(ev_start, stream);
 
     // Used nodes and their table slots.
     // mm: [0] min node, [1] max node, [4] non-finite flag, [8..11] walk
@@ -486,6 +528,8 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)U * ba.groups)),
                        dim3(ksgw::kWave * ksgw::kBuildWaves), blds, stream, ba);
     NSX_HIP(hipGetLastError(), "list build launch");
+    hipEvent_t ev_built = g_phase_events.get(1);
+    if (ev_built) (void)hipEventRecord(ev_built, stream);
     unsigned h_bad = 0;
     NSX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, stream),
             "finite check readback");
@@ -509,28 +553,30 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
             "walk occupancy");
     per_cu = std::max(per_cu, 1);
     const long long grid_max = (long long)cu_count() * per_cu;
-    // Packed counts of one launch: at most 1 GiB (larger problems launch in
-    // pair chunks).
+    // Packed counts of one launch: at most 4 GiB (larger problems launch in
+    // pair chunks); local (pair, frame) indices fit 32 bits.
     const long long chunk = std::max<long long>(
-        1, std::min<long long>(pb.P, (1LL << 30) / ((long long)n * (long long)sizeof(unsigned))));
+        1, std::min<long long>(pb.P, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
     // Tie list: up to 1/32 of a chunk's queries (ties beyond it are
     // resolved inside the walk).
     const unsigned tie_cap =
         (unsigned)std::max<long long>(1024, chunk * (long long)n / 32);
-    const long long tie_waves = (long long)cu_count() * 16;
+    const long long tie_waves = (long long)cu_count() * 8;
     unsigned *cnt = (unsigned *)workspace(
         kWsListCnt, ((size_t)chunk * n + tie_cap) * sizeof(unsigned));
-    KeyIdx *scratch = (KeyIdx *)workspace(
-        kWsTieScratch,
-        (size_t)std::max<long long>(grid_max * wpb, tie_waves) * n * sizeof(KeyIdx));
+    KeyIdx *scratch =
+        (KeyIdx *)workspace(kWsTieScratch, (size_t)grid_max * wpb * n * sizeof(KeyIdx));
     unsigned long long *stats = (unsigned long long *)(mm + 8);
     unsigned *tie_count = (unsigned *)(mm + 16);
     if (!cnt || !scratch) return kHipError;
     unsigned *tie_list = cnt + (size_t)chunk * n;
     rc = fill_u32(stats, 0u, 4, stream);
     if (rc) return rc;
+    size_t evi = 2;
     for (long long p0 = 0; p0 < pb.P; p0 += chunk) {
         const long long Pc = std::min(chunk, pb.P - p0);
+        hipEvent_t e0 = g_phase_events.get(evi++), e1 = g_phase_events.get(evi++),
+                   e2 = g_phase_events.get(evi++), e3 = g_phase_events.get(evi++);
         ksgw::WalkArgs wa;
         wa.X = X;
         wa.ld = pb.ld;
@@ -556,11 +602,14 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
         const long long grid = std::max<long long>(1, std::min<long long>(Pc, grid_max));
         rc = fill_u32(tie_count, 0u, 1, stream);
         if (rc) return rc;
+        if (e0) (void)hipEventRecord(e0, stream);
         hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream, wa);
         NSX_HIP(hipGetLastError(), "walk launch");
-        hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / 4)), dim3(256), 0, stream, wa,
-                           scratch);
+        if (e1) (void)hipEventRecord(e1, stream);
+        hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / 4)), dim3(256),
+                           (size_t)4 * n * sizeof(KeyIdx), stream, wa);
         NSX_HIP(hipGetLastError(), "tie launch");
+        if (e2) (void)hipEventRecord(e2, stream);
         ksgw::SumArgs sa;
         sa.cnt = cnt;
         sa.P = Pc;
@@ -570,14 +619,24 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
         sa.inv_k = (float)(1.0 / (float)pb.k);
         sa.mi_out = mi_out + p0;
         sa.counts_out = counts_out ? counts_out + p0 * n * 2 : nullptr;
-        hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)((Pc + 255) / 256)), dim3(256), 0,
+        const long long sum_blocks = (Pc + 64 * kSumWaves - 1) / (64 * kSumWaves);
+        hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)sum_blocks), dim3(64 * kSumWaves), 0,
                            stream, sa);
         NSX_HIP(hipGetLastError(), "psi sum launch");
+        if (e3) (void)hipEventRecord(e3, stream);
     }
     unsigned long long h_stats[2] = {0, 0};
     NSX_HIP(hipMemcpyAsync(h_stats, stats, sizeof(h_stats), hipMemcpyDeviceToHost, stream),
             "walk stats readback");
     NSX_HIP(hipStreamSynchronize(stream), "walk sync");
+    float ms = 0;
+    if (ev_start && ev_built && hipEventElapsedTime(&ms, ev_start, ev_built) == hipSuccess)
+        g_last_phase_ms[0] = ms;
+    for (size_t e = 2; e + 3 < evi; e += 4) {
+        hipEvent_t *E = &g_phase_events.ev[e];
+        for (int t = 0; t < 3; ++t)
+            if (hipEventElapsedTime(&ms, E[t], E[t + 1]) == hipSuccess) g_last_phase_ms[1 + t] += ms;
+    }
     *ran = true;
     g_last_method = 2;
     g_last_entries = (long long)h_stats[1];
@@ -717,6 +776,12 @@ int nsx_ksg_set_method(int method, int walk_waves, int walk_batch) {
 int nsx_ksg_last_method(int *method, long long *list_entries) {
     if (method) *method = g_last_method;
     if (list_entries) *list_entries = g_last_entries;
+    return kOk;
+}
+
+int nsx_ksg_last_phase_ms(double *phase_ms) {
+    if (phase_ms)
+        for (int t = 0; t < 4; ++t) phase_ms[t] = g_last_phase_ms[t];
     return kOk;
 }
 

@@ -118,7 +118,7 @@ struct SumArgs {
 
 using BuildFn = void (*)(BuildArgs);
 using WalkFn = void (*)(WalkArgs);
-using TieFn = void (*)(WalkArgs, KeyIdx *);
+using TieFn = void (*)(WalkArgs);
 
 // LDS bytes of the two kernels.
 inline size_t build_lds_bytes(int d, int n, long long rs) {
@@ -146,43 +146,72 @@ TieFn tie_d3(int k);
 
 using ksg::marg;
 
-__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
-    const unsigned lo = (unsigned)__shfl_xor((int)(unsigned)v, m);
-    const unsigned hi = (unsigned)__shfl_xor((int)(unsigned)(v >> 32), m);
+// The value of lane (lane ^ M): DPP quad permutations for M = 1, 2 (VALU),
+// ds_swizzle in bitmask mode for M = 4, 8, 16, ds_bpermute for M = 32.
+template <int M>
+__device__ __forceinline__ unsigned xor_lane(unsigned v) {
+    if constexpr (M == 1) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (M == 2) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (M < 32) {
+        return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
+    } else {
+        return (unsigned)__shfl_xor((int)v, M);
+    }
+}
+
+template <int M>
+__device__ __forceinline__ unsigned long long xor_lane_u64(unsigned long long v) {
+    const unsigned lo = xor_lane<M>((unsigned)v);
+    const unsigned hi = xor_lane<M>((unsigned)(v >> 32));
     return ((unsigned long long)hi << 32) | lo;
+}
+
+// One cross-lane bitonic stage: partner lane = lane ^ LX.
+template <int R, int LX>
+__device__ __forceinline__ void bitonic_cross(unsigned long long (&v)[R], int lane, int size) {
+    const bool lower = (lane & LX) == 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const bool asc = ((lane * R + r) & size) == 0;
+        const unsigned long long o = xor_lane_u64<LX>(v[r]);
+        const bool keep = (v[r] < o) == (lower == asc);
+        v[r] = keep ? v[r] : o;
+    }
+}
+
+// One bitonic stage (merge size SIZE, compare distance STRIDE) and the
+// stages after it, unrolled at compile time; element e = lane * R + r.
+template <int R, int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_stages(unsigned long long (&v)[R], int lane) {
+    if constexpr (STRIDE >= R) {
+        bitonic_cross<R, STRIDE / R>(v, lane, SIZE);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r & STRIDE) continue;
+            const int r2 = r | STRIDE;
+            const bool asc = ((lane * R + r) & SIZE) == 0;
+            const unsigned long long x = v[r], y = v[r2];
+            const bool sw = asc ? (x > y) : (x < y);
+            v[r] = sw ? y : x;
+            v[r2] = sw ? x : y;
+        }
+    }
+    if constexpr (STRIDE > 1) bitonic_stages<R, SIZE, STRIDE / 2>(v, lane);
+}
+
+template <int R, int SIZE>
+__device__ __forceinline__ void bitonic_sizes(unsigned long long (&v)[R], int lane) {
+    bitonic_stages<R, SIZE, SIZE / 2>(v, lane);
+    if constexpr (SIZE < kWave * R) bitonic_sizes<R, SIZE * 2>(v, lane);
 }
 
 // Ascending bitonic sort of 64 * R distinct keys; element e = lane * R + r.
 template <int R>
 __device__ __forceinline__ void bitonic_sort(unsigned long long (&v)[R], int lane) {
-#pragma unroll
-    for (int size = 2; size <= kWave * R; size <<= 1) {
-#pragma unroll
-        for (int stride = size / 2; stride > 0; stride >>= 1) {
-            if (stride >= R) {
-                const int lx = stride / R;
-                const bool lower = (lane & lx) == 0;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const bool asc = ((lane * R + r) & size) == 0;
-                    const unsigned long long o = shfl_xor_u64(v[r], lx);
-                    const bool keep = (v[r] < o) == (lower == asc);
-                    v[r] = keep ? v[r] : o;
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if (r & stride) continue;
-                    const int r2 = r | stride;
-                    const bool asc = ((lane * R + r) & size) == 0;
-                    const unsigned long long x = v[r], y = v[r2];
-                    const bool sw = asc ? (x > y) : (x < y);
-                    v[r] = sw ? y : x;
-                    v[r2] = sw ? x : y;
-                }
-            }
-        }
-    }
+    bitonic_sizes<R, 2>(v, lane);
 }
 
 // One workgroup per (node slot, row group); each wave sorts whole rows.
@@ -692,12 +721,15 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
 // std::sort order (mutual_information_cpu.cpp:48-77), takes eps over
 // positions 1..k and stores the counts from the rank tables.
 template <int D, int K>
-__global__ __launch_bounds__(256) void tie_kernel(WalkArgs a, KeyIdx *scratch_all) {
+__global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
+    // Each wave's row of n records lives in LDS: the selection's partition
+    // loops are chains of dependent reads.
+    extern __shared__ __attribute__((aligned(16))) KeyIdx tie_rows[];
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
     const int n = a.n;
-    KeyIdx *scratch = scratch_all + wave * n;
+    KeyIdx *scratch = tie_rows + (threadIdx.x / kWave) * n;
     const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
     for (long long e = wave; e < count; e += nwaves) {
         const unsigned v = a.tie_list[e];
@@ -725,7 +757,7 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a, KeyIdx *scratch_al
             r.idx = j;
             scratch[j] = r;
         }
-        ksg::wave_sync_global();
+        ksg::wave_sync();
         if (lane == 0) {
             sort_prefix(scratch, n, K + 1);
             float ex = -1.0f, ey = -1.0f;
@@ -749,7 +781,7 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a, KeyIdx *scratch_al
             const unsigned rx = a.rank[offA + jx], ry = a.rank[offB + jy];
             a.cnt[pl * n + i] = (rx - 1u) | ((ry - 1u) << 16);
         }
-        ksg::wave_sync_global();
+        ksg::wave_sync();
     }
 }
 
