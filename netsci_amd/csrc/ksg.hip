@@ -554,7 +554,7 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     per_cu = std::max(per_cu, 1);
     const long long grid_max = (long long)cu_count() * per_cu;
     // Packed counts of one launch: at most 4 GiB (larger problems launch in
-    // pair chunks); local (pair, frame) indices fit 32 bits.
+    // pair chunks), so local (pair, frame) indices stay below 2^30.
     const long long chunk = std::max<long long>(
         1, std::min<long long>(pb.P, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
     // Tie list: up to 1/32 of a chunk's queries (ties beyond it are

@@ -320,7 +320,7 @@ __device__ __forceinline__ void insert_flat(float (&h)[K + 1], int (&id)[K + 1],
         h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
     }
     id[0] = c < h[0] ? j : id[0];
-    h[0] = fminf(c, h[0]);
+    h[0] = __builtin_amdgcn_fmed3f(c, h[0], -__builtin_inff());  // min of non-NaN values
 }
 
 // (D, j) as one signed 64-bit value ordered like the list's (D, j) order:
@@ -362,13 +362,19 @@ __device__ __forceinline__ int block_entry(const uint4 &b, int q) {
     return (int)((w >> ((q & 1) * 16)) & 0xffffu);
 }
 
+// Ring buffer of local pair s (the ring holds at most 3 pairs; constant
+// divisors compile to multiplications).
+__device__ __forceinline__ int ring_slot(int s, int nbuf) {
+    return nbuf == 3 ? s % 3 : (nbuf == 2 ? (s & 1) : 0);
+}
+
 // Stage local pair s of this workgroup into ring buffer s % nbuf: both rows'
 // coordinates as per-frame records, and the two nodes' table slots.
 template <int D>
 __device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *meta, long long s,
                                            int n) {
     const long long p = a.p0 + (long long)blockIdx.x + s * (long long)gridDim.x;
-    const int b = (int)(s % a.nbuf);
+    const int b = ring_slot((int)s, a.nbuf);
     const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
     const float *xa = a.X + (long long)na * a.ld;
     const float *xb = a.X + (long long)nb * a.ld;
@@ -413,7 +419,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     const long long G = gridDim.x;
     const long long npairs =
         (long long)blockIdx.x < a.P ? (a.P - (long long)blockIdx.x + G - 1) / G : 0;
-    const long long nq = npairs * n;
+    const int nq = (int)(npairs * n);  // the host keeps P * n below 2^31
+    const double inv_n = 1.0 / (double)n;
     unsigned long long ties = 0, steps = 0;
 
     if (threadIdx.x == 0) {
@@ -429,7 +436,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     // not staged yet, 1 walks, 2 no queries left.
     int state = 0;
     bool res = false;  // a finished walk waits for its counts
-    long long q = 0;   // the lane's query (local pair q / n, frame q % n)
+    int q = 0;   // the lane's query number: frame i = q - qs * n of local pair qs
+    int qs = 0;
     int i = 0, posA = 0, posB = 0;  // entries walked in each list
     int rec = 0;  // LDS float offset of the lane's pair records (an offset, not a
                   // pointer, so every access stays an LDS instruction)
@@ -461,8 +469,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             __syncthreads();
             if (threadIdx.x == 0) {
                 int staged = ctrl[1], retired = ctrl[2], ns = 0;
-                while (retired < staged && done[retired % nbuf] == n) {
-                    done[retired % nbuf] = 0;
+                while (retired < staged && done[ring_slot(retired, nbuf)] == n) {
+                    done[ring_slot(retired, nbuf)] = 0;
                     ++retired;
                     if (staged + ns < npairs) {
                         to_stage[ns] = staged + ns;
@@ -482,7 +490,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             }
             if (ctrl[2] >= npairs) break;
         }
-        const long long avail = (long long)ctrl[1] * n;  // queries of staged pairs
+        const int avail = ctrl[1] * n;  // queries of staged pairs
         const unsigned long long want = __ballot(state == 0);
         const unsigned long long ready = __ballot(state == 3 && q < avail);
         const unsigned long long walking = __ballot(state == 1);
@@ -521,8 +529,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                                                  (unsigned)(tm >> 32),
                                                  __builtin_amdgcn_mbcnt_lo((unsigned)tm, 0u));
                 if (tie && slot < a.tie_cap) {
-                    const long long s = q / n;
-                    a.tie_list[slot] = (unsigned)(((long long)blockIdx.x + s * G) * n + i);
+                    a.tie_list[slot] = (unsigned)(((long long)blockIdx.x + qs * G) * n + i);
                     deferred = true;
                 }
                 tm = __ballot(tie && !deferred);
@@ -570,15 +577,14 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 tm &= tm - 1;
             }
             if (res) {
-                const long long s = q / n;
                 if (!deferred) {
                     pend_x = a.rank[offA + jx];
                     pend_y = a.rank[offB + jy];
-                    pend_at = ((long long)blockIdx.x + s * G) * n + i;
+                    pend_at = ((long long)blockIdx.x + qs * G) * n + i;
                     pend = true;
                 }
                 steps += (unsigned)(posA + posB);
-                atomicAdd(done + (int)(s % nbuf), 1);  // its LDS records are no longer read
+                atomicAdd(done + ring_slot(qs, nbuf), 1);  // its LDS records are no longer read
             }
             res = false;
             // (2) reserve query numbers for the lanes that want one.
@@ -592,13 +598,17 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 if (state == 0) {
                     q = base + below;
                     state = q < nq ? 3 : 2;
+                    // qs = q / n, by a double product corrected to the exact quotient.
+                    int d = (int)((double)q * inv_n);
+                    const int r = q - d * n;
+                    d = r < 0 ? d - 1 : (r >= n ? d + 1 : d);
+                    qs = d;
                 }
             }
             // (3) start the reserved queries whose pairs are staged.
             if (state == 3 && q < avail) {
-                const long long s = q / n;
-                const int b = (int)(s % nbuf);
-                i = (int)(q - s * n);
+                const int b = ring_slot(qs, nbuf);
+                i = q - qs * n;
                 rec = b * rec_floats;
                 fetch<D>(lds + rec, i, qa, qb);
 #pragma unroll
@@ -639,8 +649,9 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             } else if (posA < 8 || posB < 8 || !(hk < __builtin_inff())) {
                 useA = posA <= posB;
             } else {
-                const float ra = hk / fmaxf(lex_key(lastA), 1e-30f);
-                const float rb = hk / fmaxf(lex_key(lastB), 1e-30f);
+                // An estimate only: the hardware reciprocal is precise enough.
+                const float ra = hk * __builtin_amdgcn_rcpf(fmaxf(lex_key(lastA), 1e-30f));
+                const float rb = hk * __builtin_amdgcn_rcpf(fmaxf(lex_key(lastB), 1e-30f));
                 float pa = ra, pb = rb;
 #pragma unroll
                 for (int d = 1; d < D; ++d) {
@@ -658,7 +669,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         int posL = useA ? posA : posB;
         // The chosen list's block two ahead is requested now and used one
         // body later.
-        uint4 tmp = useA ? nxtA : nxtB;
+        uint4 tmp = {0, 0, 0, 0};  // unused when the list ends before that block
         if (state == 1 && posL + 16 < rs) tmp = load_block(a.ord + off + posL + 16);
         bool on = state == 1;
 #pragma unroll
@@ -675,17 +686,12 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 lastL = lex(keyL, j);
                 const float z = (seen || j == i) ? __builtin_inff() : fmaxf(da, db);
                 insert_flat<K>(h, id, z, j);
-                if (fmaxf(lex_key(lastL), lex_key(lastO)) > h[K]) {
-                    state = 0;
-                    res = true;
-                    on = false;
-                } else if (posL >= n) {
-                    on = false;  // this list is exhausted
-                    if (posO >= n) {
-                        state = 0;
-                        res = true;
-                    }
-                }
+                // Done when the stopping rule holds or both lists are
+                // exhausted; this body ends too when this list is.
+                const bool fin = (fmaxf(keyL, lex_key(lastO)) > h[K]) | ((posL >= n) & (posO >= n));
+                on = !fin & (posL < n);
+                state = fin ? 0 : state;
+                res = res | fin;
             }
         }
         if (useA) {
