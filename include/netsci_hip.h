@@ -109,8 +109,9 @@ int nsx_ksg_last_tie_frames(long long *count);
  * launch shape is fixed, mutual_information_gpu.cu:1055-1062). */
 int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symmetric);
 
-/* Algorithm of nsx_ksg_mi (single window): method 0 = the sorted-list path
- * whenever it applies (at most 2048 frames, finite coordinates, its tables
+/* Algorithm of nsx_ksg_mi and nsx_ksg_mi_windows (each window on its own
+ * tables, one after another): method 0 = the sorted-list path
+ * whenever it applies (at most 8192 frames whose two rows fit in LDS, finite coordinates, its tables
  * of 4*n*n bytes per used node fit in free device memory) and the all-frames
  * kernel otherwise; 1 = always the all-frames kernel; 2 = the list path or
  * an error. walk_waves (1..16) and walk_batch (1..64) shape the list path's

@@ -432,25 +432,28 @@ __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a
     }
 }
 
-// Runs the whole problem on the list path when it applies (one window of at
-// most ksgw::kMaxFrames frames, tables within the free device memory, finite
-// coordinates); otherwise returns kOk with *ran == false.
+// Runs the whole problem on the list path when it applies (windows of at
+// most ksgw::kMaxFrames frames whose pair records fit LDS, tables within the
+// free device memory, finite coordinates); otherwise returns kOk with
+// *ran == false. Windows run one after another, each with its own tables
+// (the sorted lists depend on the frame set), into mi_out[rep * P + p].
 int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_out,
               hipStream_t stream, bool *ran) {
     *ran = false;
-    const int off = pb.host_win[0], n = pb.host_win[1];
-    const bool eligible = pb.R == 1 && n <= ksgw::kMaxFrames;
     if (g_method == 1) return kOk;
+    const size_t lds_cap = 160 * 1024;
+    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    const bool eligible = pb.nmax <= ksgw::kMaxFrames && rec_bytes + 256 <= lds_cap &&
+                          (pb.R == 1 || counts_out == nullptr);
     if (!eligible) {
-        NSX_REQUIRE(g_method != 2, "ksg: the list path needs one window of at most 2048 frames");
+        NSX_REQUIRE(g_method != 2,
+                    "ksg: the list path needs windows of at most 8192 frames whose two rows "
+                    "fit in LDS (2*d*n floats <= 160 KiB)");
         return kOk;
     }
     const ksgw::WalkFn walk = walk_for(pb.d, pb.k);
     const ksgw::TieFn tiek = tie_for(pb.d, pb.k);
-    const int width = ksgw::build_width(n);
-    const ksgw::BuildFn buildk = build_for(pb.d, width);
-    NSX_REQUIRE(walk && tiek && buildk, "ksg: no list-path kernel for (d, k)");
-    const float *X = pb.X + off;
+    NSX_REQUIRE(walk && tiek, "ksg: no list-path kernel for (d, k)");
     const long long m2 = 2 * pb.P;
     for (int t = 0; t < 4; ++t) g_last_phase_ms[t] = 0;
     hipEvent_t ev_start = g_phase_events.get(0);
@@ -490,9 +493,10 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
             "slot count readback");
     NSX_HIP(hipStreamSynchronize(stream), "slot scan sync");
 
-    // Tables: ord and rank, U x n rows of rs entries each.
-    const long long rs = ((long long)n + 7) & ~7LL;
-    const size_t table_bytes = (size_t)U * n * rs * sizeof(uint16_t);
+    // Tables (ord and rank, U x n rows of rs entries each), sized for the
+    // longest window and reused by every window.
+    const long long rs_max = ((long long)pb.nmax + 7) & ~7LL;
+    const size_t table_bytes = (size_t)U * pb.nmax * rs_max * sizeof(uint16_t);
     size_t free_b = 0, total_b = 0;
     NSX_HIP(hipMemGetInfo(&free_b, &total_b), "memory info");
     const double budget = 0.85 * (double)free_b - 512.0 * (1 << 20);
@@ -508,134 +512,166 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     } free_tables{tables};
     uint16_t *ord = tables, *rank = tables + table_bytes / sizeof(uint16_t);
 
-    unsigned *bad = (unsigned *)(mm + 4);
-    rc = fill_u32(bad, 0u, 1, stream);
-    if (rc) return rc;
-    ksgw::BuildArgs ba;
-    ba.X = X;
-    ba.ld = pb.ld;
-    ba.dstride = pb.dstride;
-    ba.n = n;
-    ba.rs = rs;
-    ba.node_of = node_of;
-    ba.groups = (int)std::max<long long>(
-        1, std::min<long long>((n + ksgw::kBuildWaves - 1) / ksgw::kBuildWaves,
-                               (8LL * cu_count() + U - 1) / std::max(U, 1)));
-    ba.ord = ord;
-    ba.rank = rank;
-    ba.bad = bad;
-    const size_t blds = ksgw::build_lds_bytes(pb.d, n, rs);
-    hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)U * ba.groups)),
-                       dim3(ksgw::kWave * ksgw::kBuildWaves), blds, stream, ba);
-    NSX_HIP(hipGetLastError(), "list build launch");
-    hipEvent_t ev_built = g_phase_events.get(1);
-    if (ev_built) (void)hipEventRecord(ev_built, stream);
-    unsigned h_bad = 0;
-    NSX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, stream),
-            "finite check readback");
-    NSX_HIP(hipStreamSynchronize(stream), "list build sync");
-    if (h_bad) {
-        NSX_REQUIRE(g_method != 2, "ksg: the list path needs finite coordinates");
-        return kOk;
-    }
-
-    // Walk: one workgroup of g_walk_wpb waves per CU, as many pair buffers
-    // (up to 3) as fit in its LDS.
+    // Per-call buffers sized for the longest window.
     const int wpb = std::min(g_walk_wpb, ksgw::walk_max_waves(pb.k));
-    const size_t rec_bytes = (size_t)2 * pb.d * n * sizeof(float);
-    const size_t lds_cap = 160 * 1024;
-    int nbuf = (int)std::min<size_t>(3, (lds_cap - 256) / rec_bytes);
-    NSX_REQUIRE(nbuf >= 1, "ksg: the list path's pair records exceed LDS");
-    const size_t wlds = ksgw::walk_lds_bytes(pb.d, n, nbuf);
     int per_cu = 0;
-    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)walk,
-                                                         ksgw::kWave * wpb, wlds),
+    const int nbuf_max = (int)std::min<size_t>(3, (lds_cap - 256) / rec_bytes);
+    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void *)walk, ksgw::kWave * wpb,
+                ksgw::walk_lds_bytes(pb.d, pb.nmax, nbuf_max)),
             "walk occupancy");
     per_cu = std::max(per_cu, 1);
     const long long grid_max = (long long)cu_count() * per_cu;
     // Packed counts of one launch: at most 4 GiB (larger problems launch in
     // pair chunks), so local (pair, frame) indices stay below 2^30.
     const long long chunk = std::max<long long>(
-        1, std::min<long long>(pb.P, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
+        1, std::min<long long>(pb.P, (1LL << 32) / ((long long)pb.nmax * (long long)sizeof(unsigned))));
     // Tie list: up to 1/32 of a chunk's queries (ties beyond it are
     // resolved inside the walk).
     const unsigned tie_cap =
-        (unsigned)std::max<long long>(1024, chunk * (long long)n / 32);
+        (unsigned)std::max<long long>(1024, chunk * (long long)pb.nmax / 32);
     const long long tie_waves = (long long)cu_count() * 8;
-    unsigned *cnt = (unsigned *)workspace(
-        kWsListCnt, ((size_t)chunk * n + tie_cap) * sizeof(unsigned));
+    // One row of margin: a shorter window's chunk x n can pass chunk x nmax
+    // by less than nmax.
+    const size_t cnt_words = (size_t)chunk * pb.nmax + pb.nmax;
+    unsigned *cnt = (unsigned *)workspace(kWsListCnt, (cnt_words + tie_cap) * sizeof(unsigned));
     KeyIdx *scratch =
-        (KeyIdx *)workspace(kWsTieScratch, (size_t)grid_max * wpb * n * sizeof(KeyIdx));
+        (KeyIdx *)workspace(kWsTieScratch, (size_t)grid_max * wpb * pb.nmax * sizeof(KeyIdx));
     unsigned long long *stats = (unsigned long long *)(mm + 8);
     unsigned *tie_count = (unsigned *)(mm + 16);
+    unsigned *bad = (unsigned *)(mm + 4);
     if (!cnt || !scratch) return kHipError;
-    unsigned *tie_list = cnt + (size_t)chunk * n;
+    unsigned *tie_list = cnt + cnt_words;
     rc = fill_u32(stats, 0u, 4, stream);
     if (rc) return rc;
-    size_t evi = 2;
-    for (long long p0 = 0; p0 < pb.P; p0 += chunk) {
-        const long long Pc = std::min(chunk, pb.P - p0);
-        hipEvent_t e0 = g_phase_events.get(evi++), e1 = g_phase_events.get(evi++),
-                   e2 = g_phase_events.get(evi++), e3 = g_phase_events.get(evi++);
-        ksgw::WalkArgs wa;
-        wa.X = X;
-        wa.ld = pb.ld;
-        wa.dstride = pb.dstride;
-        wa.ab = pb.ab;
-        wa.p0 = p0;
-        wa.P = Pc;
-        wa.n = n;
-        wa.k = pb.k;
-        wa.ord = ord;
-        wa.rank = rank;
-        wa.rs = rs;
-        wa.slot_of = slot_of;
-        wa.scratch = scratch;
-        wa.cnt = cnt;
-        wa.stats = stats;
-        wa.tie_list = tie_list;
-        wa.tie_count = tie_count;
-        wa.tie_cap = tie_cap;
-        wa.nbuf = nbuf;
-        wa.batch = g_walk_batch;
-        wa.checkpoint = 8;
-        const long long grid = std::max<long long>(1, std::min<long long>(Pc, grid_max));
-        rc = fill_u32(tie_count, 0u, 1, stream);
+
+    size_t evi = 1;
+    struct Mark {
+        int phase;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;  // (phase, start, stop) event pairs on the stream
+    for (int w = 0; w < pb.R; ++w) {
+        const int off = pb.host_win[w], n = pb.host_win[pb.R + w], rep = pb.host_win[2 * pb.R + w];
+        const float *X = pb.X + off;
+        const long long rs = ((long long)n + 7) & ~7LL;
+        float *out = mi_out + (long long)rep * pb.P;
+
+        // (1) tables for this window's frames.
+        const int width = ksgw::build_width(n);
+        const ksgw::BuildFn buildk = build_for(pb.d, width);
+        NSX_REQUIRE(buildk, "ksg: no list-path table kernel for this frame count");
+        rc = fill_u32(bad, 0u, 1, stream);
         if (rc) return rc;
-        if (e0) (void)hipEventRecord(e0, stream);
-        hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream, wa);
-        NSX_HIP(hipGetLastError(), "walk launch");
-        if (e1) (void)hipEventRecord(e1, stream);
-        hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / 4)), dim3(256),
-                           (size_t)4 * n * sizeof(KeyIdx), stream, wa);
-        NSX_HIP(hipGetLastError(), "tie launch");
-        if (e2) (void)hipEventRecord(e2, stream);
-        ksgw::SumArgs sa;
-        sa.cnt = cnt;
-        sa.P = Pc;
-        sa.n = n;
-        sa.k = pb.k;
-        sa.psi = d_psi;
-        sa.inv_k = (float)(1.0 / (float)pb.k);
-        sa.mi_out = mi_out + p0;
-        sa.counts_out = counts_out ? counts_out + p0 * n * 2 : nullptr;
-        const long long sum_blocks = (Pc + 64 * kSumWaves - 1) / (64 * kSumWaves);
-        hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)sum_blocks), dim3(64 * kSumWaves), 0,
-                           stream, sa);
-        NSX_HIP(hipGetLastError(), "psi sum launch");
-        if (e3) (void)hipEventRecord(e3, stream);
+        hipEvent_t b0 = g_phase_events.get(evi++), b1 = g_phase_events.get(evi++);
+        if (b0) (void)hipEventRecord(b0, stream);
+        ksgw::BuildArgs ba;
+        ba.X = X;
+        ba.ld = pb.ld;
+        ba.dstride = pb.dstride;
+        ba.n = n;
+        ba.rs = rs;
+        ba.node_of = node_of;
+        const int rows_per_pass = ksgw::kBuildWaves / ksgw::build_row_waves(width);
+        ba.groups = (int)std::max<long long>(
+            1, std::min<long long>((n + rows_per_pass - 1) / rows_per_pass,
+                                   (8LL * cu_count() + U - 1) / std::max(U, 1)));
+        ba.ord = ord;
+        ba.rank = rank;
+        ba.bad = bad;
+        hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)U * ba.groups)),
+                           dim3(ksgw::kWave * ksgw::kBuildWaves),
+                           ksgw::build_lds_bytes(pb.d, n, rs), stream, ba);
+        NSX_HIP(hipGetLastError(), "list build launch");
+        if (b1) (void)hipEventRecord(b1, stream);
+        marks.push_back({0, b0, b1});
+        if (w == 0) marks.push_back({0, ev_start, b0});  // the node map
+        unsigned h_bad = 0;
+        NSX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, stream),
+                "finite check readback");
+        NSX_HIP(hipStreamSynchronize(stream), "list build sync");
+        if (h_bad) {
+            NSX_REQUIRE(g_method != 2, "ksg: the list path needs finite coordinates");
+            return kOk;
+        }
+
+        // (2) walk, ties, psi sums, per chunk of pairs.
+        const size_t wrec = (size_t)2 * pb.d * n * sizeof(float);
+        const int nbuf = (int)std::min<size_t>(3, (lds_cap - 256) / wrec);
+        const size_t wlds = ksgw::walk_lds_bytes(pb.d, n, nbuf);
+        const long long wchunk = std::max<long long>(
+            1, std::min<long long>(pb.P, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
+        const unsigned wtie_cap =
+            (unsigned)std::min<long long>(tie_cap, std::max<long long>(1024, wchunk * (long long)n / 32));
+        for (long long p0 = 0; p0 < pb.P; p0 += wchunk) {
+            const long long Pc = std::min(wchunk, pb.P - p0);
+            hipEvent_t e0 = g_phase_events.get(evi++), e1 = g_phase_events.get(evi++),
+                       e2 = g_phase_events.get(evi++), e3 = g_phase_events.get(evi++);
+            ksgw::WalkArgs wa;
+            wa.X = X;
+            wa.ld = pb.ld;
+            wa.dstride = pb.dstride;
+            wa.ab = pb.ab;
+            wa.p0 = p0;
+            wa.P = Pc;
+            wa.n = n;
+            wa.k = pb.k;
+            wa.ord = ord;
+            wa.rank = rank;
+            wa.rs = rs;
+            wa.slot_of = slot_of;
+            wa.scratch = scratch;
+            wa.cnt = cnt;
+            wa.stats = stats;
+            wa.tie_list = tie_list;
+            wa.tie_count = tie_count;
+            wa.tie_cap = wtie_cap;
+            wa.nbuf = nbuf;
+            wa.batch = g_walk_batch;
+            wa.checkpoint = 8;
+            const long long grid = std::max<long long>(1, std::min<long long>(Pc, grid_max));
+            rc = fill_u32(tie_count, 0u, 1, stream);
+            if (rc) return rc;
+            if (e0) (void)hipEventRecord(e0, stream);
+            hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream,
+                               wa);
+            NSX_HIP(hipGetLastError(), "walk launch");
+            if (e1) (void)hipEventRecord(e1, stream);
+            // Up to 4 waves per workgroup, each with an n-record LDS row.
+            const int tie_wpb = (int)std::max<size_t>(
+                1, std::min<size_t>(4, lds_cap / ((size_t)n * sizeof(KeyIdx))));
+            hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / tie_wpb)),
+                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * n * sizeof(KeyIdx),
+                               stream, wa);
+            NSX_HIP(hipGetLastError(), "tie launch");
+            if (e2) (void)hipEventRecord(e2, stream);
+            ksgw::SumArgs sa;
+            sa.cnt = cnt;
+            sa.P = Pc;
+            sa.n = n;
+            sa.k = pb.k;
+            sa.psi = d_psi;
+            sa.inv_k = (float)(1.0 / (float)pb.k);
+            sa.mi_out = out + p0;
+            sa.counts_out = counts_out ? counts_out + p0 * n * 2 : nullptr;
+            const long long sum_blocks = (Pc + 64 * kSumWaves - 1) / (64 * kSumWaves);
+            hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)sum_blocks), dim3(64 * kSumWaves),
+                               0, stream, sa);
+            NSX_HIP(hipGetLastError(), "psi sum launch");
+            if (e3) (void)hipEventRecord(e3, stream);
+            marks.push_back({1, e0, e1});
+            marks.push_back({2, e1, e2});
+            marks.push_back({3, e2, e3});
+        }
     }
     unsigned long long h_stats[2] = {0, 0};
     NSX_HIP(hipMemcpyAsync(h_stats, stats, sizeof(h_stats), hipMemcpyDeviceToHost, stream),
             "walk stats readback");
     NSX_HIP(hipStreamSynchronize(stream), "walk sync");
-    float ms = 0;
-    if (ev_start && ev_built && hipEventElapsedTime(&ms, ev_start, ev_built) == hipSuccess)
-        g_last_phase_ms[0] = ms;
-    for (size_t e = 2; e + 3 < evi; e += 4) {
-        hipEvent_t *E = &g_phase_events.ev[e];
-        for (int t = 0; t < 3; ++t)
-            if (hipEventElapsedTime(&ms, E[t], E[t + 1]) == hipSuccess) g_last_phase_ms[1 + t] += ms;
+    for (const Mark &m : marks) {
+        float ms = 0;
+        if (m.a && m.b && hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess)
+            g_last_phase_ms[m.phase] += ms;
     }
     *ran = true;
     g_last_method = 2;
@@ -654,7 +690,7 @@ int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t str
     g_last_method = 0;
     g_last_entries = 0;
     if (pb.P == 0) return kOk;
-    if (g_method != 1 && pb.R == 1) {
+    if (g_method != 1) {
         std::vector<float> psi(pb.nmax + 1);
         psi_table(psi.data(), pb.nmax);
         float *d_psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));

@@ -63,7 +63,7 @@ namespace nsx {
 namespace ksgw {
 
 constexpr int kWave = 64;
-constexpr int kMaxFrames = 2048;  // 32 sort keys per lane
+constexpr int kMaxFrames = 8192;  // 32 sort keys per lane, up to 4 waves per row
 constexpr int kBuildWaves = 4;
 constexpr int kMaxWalkWaves = 16;
 
@@ -122,14 +122,19 @@ using TieFn = void (*)(WalkArgs);
 
 // LDS bytes of the two kernels.
 inline size_t build_lds_bytes(int d, int n, long long rs) {
+    if (n > 2048) return (size_t)kBuildWaves * 2048 * 8;  // the wide kernel's exchange rows
     return (((size_t)d * n * 4 + 15) & ~(size_t)15) + (size_t)kBuildWaves * rs * 2;
 }
 inline size_t walk_lds_bytes(int d, int n, int nbuf) {
     return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 4 * nbuf) * sizeof(int);
 }
 
-// Sort keys per lane for n frames (8, 16 or 32).
-inline int build_width(int n) { return n <= 512 ? 8 : n <= 1024 ? 16 : 32; }
+// Sort elements per wave-lane for n frames: 8, 16 or 32 for one wave per
+// row; 64 and 128 mean 2 and 4 waves per row (32 each).
+inline int build_width(int n) {
+    return n <= 512 ? 8 : n <= 1024 ? 16 : n <= 2048 ? 32 : n <= 4096 ? 64 : 128;
+}
+inline int build_row_waves(int width) { return width <= 32 ? 1 : width / 32; }
 
 // Kernel instances, one translation unit per coordinate dimension.
 BuildFn build_d1(int width);
@@ -168,13 +173,16 @@ __device__ __forceinline__ unsigned long long xor_lane_u64(unsigned long long v)
     return ((unsigned long long)hi << 32) | lo;
 }
 
-// One cross-lane bitonic stage: partner lane = lane ^ LX.
+// One cross-lane bitonic stage: partner lane = lane ^ LX. ebase is the
+// element index of the wave's first element (its 2048-element block in the
+// row), so the merge direction is that of the whole row's sort.
 template <int R, int LX>
-__device__ __forceinline__ void bitonic_cross(unsigned long long (&v)[R], int lane, int size) {
+__device__ __forceinline__ void bitonic_cross(unsigned long long (&v)[R], int lane, int size,
+                                              int ebase) {
     const bool lower = (lane & LX) == 0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const bool asc = ((lane * R + r) & size) == 0;
+        const bool asc = ((ebase + lane * R + r) & size) == 0;
         const unsigned long long o = xor_lane_u64<LX>(v[r]);
         const bool keep = (v[r] < o) == (lower == asc);
         v[r] = keep ? v[r] : o;
@@ -184,34 +192,66 @@ __device__ __forceinline__ void bitonic_cross(unsigned long long (&v)[R], int la
 // One bitonic stage (merge size SIZE, compare distance STRIDE) and the
 // stages after it, unrolled at compile time; element e = lane * R + r.
 template <int R, int SIZE, int STRIDE>
-__device__ __forceinline__ void bitonic_stages(unsigned long long (&v)[R], int lane) {
+__device__ __forceinline__ void bitonic_stages(unsigned long long (&v)[R], int lane,
+                                               int ebase = 0) {
     if constexpr (STRIDE >= R) {
-        bitonic_cross<R, STRIDE / R>(v, lane, SIZE);
+        bitonic_cross<R, STRIDE / R>(v, lane, SIZE, ebase);
     } else {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (r & STRIDE) continue;
             const int r2 = r | STRIDE;
-            const bool asc = ((lane * R + r) & SIZE) == 0;
+            const bool asc = ((ebase + lane * R + r) & SIZE) == 0;
             const unsigned long long x = v[r], y = v[r2];
             const bool sw = asc ? (x > y) : (x < y);
             v[r] = sw ? y : x;
             v[r2] = sw ? x : y;
         }
     }
-    if constexpr (STRIDE > 1) bitonic_stages<R, SIZE, STRIDE / 2>(v, lane);
+    if constexpr (STRIDE > 1) bitonic_stages<R, SIZE, STRIDE / 2>(v, lane, ebase);
 }
 
 template <int R, int SIZE>
-__device__ __forceinline__ void bitonic_sizes(unsigned long long (&v)[R], int lane) {
-    bitonic_stages<R, SIZE, SIZE / 2>(v, lane);
-    if constexpr (SIZE < kWave * R) bitonic_sizes<R, SIZE * 2>(v, lane);
+__device__ __forceinline__ void bitonic_sizes(unsigned long long (&v)[R], int lane,
+                                              int ebase = 0) {
+    bitonic_stages<R, SIZE, SIZE / 2>(v, lane, ebase);
+    if constexpr (SIZE < kWave * R) bitonic_sizes<R, SIZE * 2>(v, lane, ebase);
 }
 
 // Ascending bitonic sort of 64 * R distinct keys; element e = lane * R + r.
+// With ebase != 0 the wave sorts its block in the direction a row-wide sort
+// gives it (descending when bit log2(64 R) of ebase is set).
 template <int R>
-__device__ __forceinline__ void bitonic_sort(unsigned long long (&v)[R], int lane) {
-    bitonic_sizes<R, 2>(v, lane);
+__device__ __forceinline__ void bitonic_sort(unsigned long long (&v)[R], int lane,
+                                             int ebase = 0) {
+    bitonic_sizes<R, 2>(v, lane, ebase);
+}
+
+// Row-wide merges after every wave sorted its 2048-element block: sizes
+// 4096 .. 2048 W. Strides of 2048 and more exchange through LDS (xr: this
+// row's W x 2048 elements); smaller ones stay in the wave.
+template <int W, int SIZE>
+__device__ __forceinline__ void wide_merges(unsigned long long (&v)[32], int lane, int ebase,
+                                            unsigned long long *xr) {
+    if constexpr (SIZE <= 2048 * W) {
+#pragma unroll
+        for (int stride = SIZE / 2; stride >= 2048; stride >>= 1) {
+#pragma unroll
+            for (int r = 0; r < 32; ++r) xr[ebase + lane * 32 + r] = v[r];
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const int e = ebase + lane * 32 + r;
+                const unsigned long long o = xr[e ^ stride];
+                const bool asc = (e & SIZE) == 0;
+                const bool keep = (v[r] < o) == (((e & stride) == 0) == asc);
+                v[r] = keep ? v[r] : o;
+            }
+            __syncthreads();
+        }
+        bitonic_stages<32, SIZE, 1024>(v, lane, ebase);
+        wide_merges<W, SIZE * 2>(v, lane, ebase, xr);
+    }
 }
 
 // One workgroup per (node slot, row group); each wave sorts whole rows.
@@ -305,6 +345,117 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
         for (long long c = (long long)lane * 8; c < a.rs; c += kWave * 8)
             *reinterpret_cast<uint4 *>(rrow + c) = *reinterpret_cast<const uint4 *>(st + c);
         ksg::wave_sync();
+    }
+}
+
+// Rows of 2048 < n <= 2048 W frames: W waves sort one row together (each a
+// 2048-element block, 32 per lane), merging across waves through LDS. A
+// workgroup of kBuildWaves waves holds kBuildWaves / W rows; coordinates are
+// read from global memory (the node's row stays in L2).
+template <int D, int W>
+__global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(BuildArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned long long xch[];
+    __shared__ unsigned first_key[kBuildWaves];
+    __shared__ unsigned wave_min[kBuildWaves];
+    constexpr int kRows = kBuildWaves / W;
+    const int n = a.n;
+    const int slot = blockIdx.x / a.groups;
+    const int g = blockIdx.x - slot * a.groups;
+    const float *x = a.X + (long long)a.node_of[slot] * a.ld;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wib = threadIdx.x / kWave;
+    const int rg = wib / W, wl = wib - rg * W;  // row in the workgroup, block in the row
+    const int ebase = wl * 2048;
+    bool bad = false;
+    for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
+        const int d = t / n, f = t - d * n;
+        bad |= !__builtin_isfinite(x[d * a.dstride + f]);
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(a.bad, 1u);
+    unsigned long long *xr = xch + rg * (W * 2048);
+    uint16_t *st = reinterpret_cast<uint16_t *>(xr);  // rank staging after the sort
+    for (int base = g * kRows; base < n; base += a.groups * kRows) {
+        const int i = base + rg;
+        const bool row = i < n;
+        float q[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) q[d] = row ? x[d * a.dstride + i] : 0.0f;
+        unsigned long long v[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const int e = ebase + lane * 32 + r;
+            if (row && e < n) {
+                float c[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) c[d] = x[d * a.dstride + e];
+                const float key = marg<D>(q, c);
+                v[r] = ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)e;
+            } else {
+                v[r] = ~0ull;
+            }
+        }
+        bitonic_sort<32>(v, lane, ebase);
+        wide_merges<W, 4096>(v, lane, ebase, xr);
+
+        // ord row.
+        uint16_t *orow = a.ord + ((long long)slot * n + (row ? i : 0)) * a.rs;
+#pragma unroll
+        for (int r0 = 0; r0 < 32; r0 += 8) {
+            const long long e0 = (long long)ebase + lane * 32 + r0;
+            if (row && e0 < a.rs) {
+                uint4 w;
+                w.x = (unsigned)(v[r0 + 0] & 0xffff) | ((unsigned)(v[r0 + 1] & 0xffff) << 16);
+                w.y = (unsigned)(v[r0 + 2] & 0xffff) | ((unsigned)(v[r0 + 3] & 0xffff) << 16);
+                w.z = (unsigned)(v[r0 + 4] & 0xffff) | ((unsigned)(v[r0 + 5] & 0xffff) << 16);
+                w.w = (unsigned)(v[r0 + 6] & 0xffff) | ((unsigned)(v[r0 + 7] & 0xffff) << 16);
+                *reinterpret_cast<uint4 *>(orow + e0) = w;
+            }
+        }
+
+        // Upper-bound ranks: the first end of an equal-key run at or after
+        // each position, across the row's waves.
+        if (lane == 0) first_key[wib] = (unsigned)(v[0] >> 32);
+        __syncthreads();
+        constexpr unsigned kBig = 0x7fffffffu;
+        const unsigned next_wave_first = wl + 1 < W ? first_key[wib + 1] : 0xffffffffu;
+        const unsigned next0 = (unsigned)__shfl_down((int)(unsigned)(v[0] >> 32), 1);
+        unsigned sfx[32];
+        unsigned m = kBig;
+#pragma unroll
+        for (int r = 31; r >= 0; --r) {
+            const unsigned kr = (unsigned)(v[r] >> 32);
+            const unsigned kn = (r == 31) ? (lane == kWave - 1 ? next_wave_first : next0)
+                                          : (unsigned)(v[r + 1] >> 32);
+            const unsigned end = kn != kr ? (unsigned)(ebase + lane * 32 + r + 1) : kBig;
+            m = min(m, end);
+            sfx[r] = m;
+        }
+        unsigned t = m;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const unsigned o = (unsigned)__shfl_down((int)t, off);
+            t = lane + off < kWave ? min(t, o) : t;
+        }
+        if (lane == 0) wave_min[wib] = t;
+        __syncthreads();
+        unsigned later = kBig;  // minimum over the row's later waves
+        for (int w2 = wl + 1; w2 < W; ++w2) later = min(later, wave_min[rg * W + w2]);
+        unsigned after = (unsigned)__shfl_down((int)t, 1);
+        after = lane == kWave - 1 ? later : min(after, later);
+        if (row) {
+#pragma unroll
+            for (int r = 0; r < 32; ++r) {
+                const int e = ebase + lane * 32 + r;
+                if (e < n) st[(unsigned)(v[r] & 0xffff)] = (uint16_t)min(sfx[r], after);
+            }
+        }
+        __syncthreads();
+        if (row) {
+            uint16_t *rrow = a.rank + ((long long)slot * n + i) * a.rs;
+            for (long long c = ((long long)wl * kWave + lane) * 8; c < a.rs; c += W * kWave * 8)
+                *reinterpret_cast<uint4 *>(rrow + c) = *reinterpret_cast<const uint4 *>(st + c);
+        }
+        __syncthreads();
     }
 }
 
@@ -833,6 +984,8 @@ BuildFn build_lookup(int width) {
         case 8: return list_build_kernel<D, 8>;
         case 16: return list_build_kernel<D, 16>;
         case 32: return list_build_kernel<D, 32>;
+        case 64: return list_build_wide_kernel<D, 2>;
+        case 128: return list_build_wide_kernel<D, 4>;
         default: return nullptr;
     }
 }

@@ -260,18 +260,22 @@ def test_list_path_sizes(method_variant, n, k):
 
 
 def test_list_path_limits_and_fallback(method_variant):
-    """Above 2048 frames, or with a non-finite coordinate, the automatic
+    """Above 8192 frames, or with a non-finite coordinate, the automatic
     choice runs the all-frames kernel; METHOD_LISTS raises instead."""
     if method_variant != "lists":
         pytest.skip("method selection is exercised in the lists variant")
-    d, k = 3, 6
-    _flush("list fallback n=2049")
-    X3, _ = synth.calpha_trajectory(N=2, n=2049, seed=5)
-    _check_pairs(X3, np.array([[0, 1]], np.int32), k, 2049, d, 1)
+    d, k = 1, 4
+    _flush("list fallback n=8193")
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((2, 8193)).astype(np.float32)
+    ab = np.array([[0, 1]], np.int32)
+    _gpu_counts(X, ab, k, 8193, d)
+    assert abi.last_method()[0] == 1
     abi.set_method(abi.METHOD_LISTS)
-    with pytest.raises(RuntimeError, match="2048"):
-        _gpu_counts(X3, np.array([[0, 1]], np.int32), k, 2049, d)
+    with pytest.raises(RuntimeError, match="8192"):
+        _gpu_counts(X, ab, k, 8193, d)
     _flush("list fallback non-finite")
+    d, k = 3, 6
     X3, _ = synth.calpha_trajectory(N=2, n=300, seed=6)
     X3[1, 17] = np.inf
     with pytest.raises(RuntimeError, match="finite"):
@@ -281,6 +285,48 @@ def test_list_path_limits_and_fallback(method_variant):
     assert abi.last_method()[0] == 2  # node 1 (the non-finite row) is not used
     mi, _, _ = _gpu_counts(X3, np.array([[0, 1]], np.int32), k, 300, d)
     assert abi.last_method()[0] == 1
+
+
+@pytest.mark.parametrize("d,n,k", [(3, 2049, 6), (3, 2055, 6), (3, 3000, 4), (3, 4096, 6),
+                                   (3, 4097, 6), (3, 5000, 6), (3, 6000, 3), (2, 7000, 6),
+                                   (1, 8192, 4)])
+def test_list_path_wide_tables(method_variant, d, n, k):
+    """Rows sorted by 2 or 4 waves together (n > 2048): the cross-wave merges,
+    ranks whose equal-key runs cross a wave boundary, and one pair per
+    workgroup in LDS."""
+    if method_variant != "lists":
+        pytest.skip("wide tables only exist on the list path")
+    _flush(f"wide tables d={d} n={n} k={k}")
+    rng = np.random.default_rng(n + d)
+    if d == 3:
+        X, _ = synth.calpha_trajectory(N=3, n=n, seed=n)
+    else:  # a coarse grid: many equal keys, runs across wave blocks
+        X = (np.round(rng.standard_normal((3, d * n)) * 6) / 6).astype(np.float32)
+    ab = np.array([[0, 1], [1, 2]], np.int32)
+    _check_pairs(X, ab, k, n, d, 2)
+
+
+def test_list_path_long_windows(method_variant):
+    """Bootstrap windows of uneven lengths up to 4100 frames go through the
+    list path one window at a time (each with its own tables); every window
+    equals the oracle on its frames."""
+    if method_variant != "lists":
+        pytest.skip("windows on the list path")
+    N, nT, d, k = 3, 4200, 3, 6
+    _flush("long windows on the list path")
+    X, _ = synth.calpha_trajectory(N=N, n=nT, seed=31)
+    ab = synth.all_pairs(N)
+    win = np.array([[0, 4100], [100, 2500], [7, 3001], [4000, 200]], np.int32)
+    dX = abi.DeviceArray.from_numpy(X)
+    dab = abi.DeviceArray.from_numpy(ab)
+    dmi = abi.DeviceArray((len(win), len(ab)), np.float32)
+    abi.ksg_mi_windows(dX, X.shape[1], nT, d, win, dab, len(ab), k, dmi)
+    assert abi.last_method()[0] == 2
+    got = dmi.numpy()
+    for r, (off, ln) in enumerate(win):
+        Xw = np.concatenate([X[:, di * nT + off: di * nT + off + ln] for di in range(d)], axis=1)
+        want = oracle.pairs(Xw, ab, k, int(ln), d, gc=False)
+        np.testing.assert_array_equal(got[r], want)
 
 
 @pytest.mark.parametrize("waves,batch", [(1, 1), (2, 64), (8, 16), (4, 3)])
