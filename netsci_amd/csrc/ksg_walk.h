@@ -254,6 +254,80 @@ __device__ __forceinline__ void wide_merges(unsigned long long (&v)[32], int lan
     }
 }
 
+// ---- rows of n <= 2048 frames: one wave per row, 32-bit sort keys ----------
+//
+// A row's elements are packed as (the key's float bits >> 10) << 11 | j: the
+// truncated key keeps the order of the non-negative keys (equal truncations
+// only for keys within 2^-13 relative of each other) and j < 2048 fits the
+// low bits. These 32-bit values are sorted with every merge ascending (the
+// first stage of a merge compares mirrored positions), so a compare-exchange
+// is one v_min_u32 and one v_max_u32. The exact (key, j) order inside runs of
+// equal truncated keys is then restored by odd-even transposition on the exact
+// keys, repeated until no pair swaps.
+
+// Partner value of the lane (lane ^ M), for any M < 64.
+template <int M>
+__device__ __forceinline__ unsigned xor_lane_any(unsigned v) {
+    if constexpr (M == 1 || M == 2 || M == 4 || M == 8 || M == 16 || M == 32) {
+        return xor_lane<M>(v);
+    } else if constexpr (M == 3) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x1B, 0xF, 0xF, false);
+    } else if constexpr (M < 32) {
+        return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
+    } else {
+        return (unsigned)__shfl_xor((int)v, M);
+    }
+}
+
+// Half-cleaner stages STRIDE, STRIDE/2, .., 1 (all ascending).
+template <int R, int STRIDE>
+__device__ __forceinline__ void asc_stages(unsigned (&v)[R], int lane) {
+    if constexpr (STRIDE >= R) {
+        constexpr int LX = STRIDE / R;
+        const bool lower = (lane & LX) == 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned o = xor_lane<LX>(v[r]);
+            v[r] = lower ? min(v[r], o) : max(v[r], o);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (r & STRIDE) continue;
+            const unsigned x = v[r], y = v[r | STRIDE];
+            v[r] = min(x, y);
+            v[r | STRIDE] = max(x, y);
+        }
+    }
+    if constexpr (STRIDE > 1) asc_stages<R, STRIDE / 2>(v, lane);
+}
+
+// Merges of size SIZE .. 64 R: a mirror stage, then the half-cleaners.
+template <int R, int SIZE>
+__device__ __forceinline__ void mirror_merges(unsigned (&v)[R], int lane) {
+    if constexpr (SIZE <= R) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int r2 = r ^ (SIZE - 1);
+            if (r2 <= r) continue;
+            const unsigned x = v[r], y = v[r2];
+            v[r] = min(x, y);
+            v[r2] = max(x, y);
+        }
+    } else {
+        // Element (lane, r) pairs with (lane ^ M, R - 1 - r).
+        constexpr int M = SIZE / R - 1;
+        const bool lower = (lane & ((M + 1) >> 1)) == 0;
+        unsigned o[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) o[r] = xor_lane_any<M>(v[R - 1 - r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = lower ? min(v[r], o[r]) : max(v[r], o[r]);
+    }
+    if constexpr (SIZE >= 4) asc_stages<R, SIZE / 4>(v, lane);
+    if constexpr (SIZE < kWave * R) mirror_merges<R, SIZE * 2>(v, lane);
+}
+
 // One workgroup per (node slot, row group); each wave sorts whole rows.
 template <int D, int R>
 __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArgs a) {
@@ -276,11 +350,12 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
     uint16_t *st = reinterpret_cast<uint16_t *>(
                        reinterpret_cast<char *>(lds) + ((D * n * 4 + 15) & ~15)) +
                    (long long)wib * a.rs;
+    constexpr unsigned kPad = 0xffffffffu;  // above every packed finite key
     for (int i = g * kBuildWaves + wib; i < n; i += a.groups * kBuildWaves) {
         float q[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = lds[d * n + i];
-        unsigned long long v[R];
+        unsigned v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int e = lane * R + r;
@@ -288,13 +363,65 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
                 float c[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) c[d] = lds[d * n + e];
-                const float key = marg<D>(q, c);
-                v[r] = ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)e;
+                v[r] = ((__float_as_uint(marg<D>(q, c)) >> 10) << 11) | (unsigned)e;
             } else {
-                v[r] = ~0ull;
+                v[r] = kPad;
             }
         }
-        bitonic_sort<R>(v, lane);
+        mirror_merges<R, 2>(v, lane);
+
+        // Exact order: the elements as (exact key bits << 32 | j), then
+        // odd-even transposition until no pair swaps. Pairs whose truncated
+        // keys differ are already in exact order, so only equal-truncation
+        // runs (typically of 2 or 3) move; the loop ends after as many passes
+        // as the longest run.
+        unsigned long long ex[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (v[r] != kPad) {
+                float c[D];
+                const int j = (int)(v[r] & 0x7ffu);
+#pragma unroll
+                for (int d = 0; d < D; ++d) c[d] = lds[d * n + j];
+                ex[r] = ((unsigned long long)__float_as_uint(marg<D>(q, c)) << 32) | (unsigned)j;
+            } else {
+                ex[r] = ~0ull;
+            }
+        }
+        for (;;) {
+            bool swapped = false;
+#pragma unroll
+            for (int r = 0; r + 1 < R; r += 2) {
+                const unsigned long long x = ex[r], y = ex[r + 1];
+                swapped |= x > y;
+                ex[r] = x < y ? x : y;
+                ex[r + 1] = x < y ? y : x;
+            }
+#pragma unroll
+            for (int r = 1; r + 1 < R; r += 2) {
+                const unsigned long long x = ex[r], y = ex[r + 1];
+                swapped |= x > y;
+                ex[r] = x < y ? x : y;
+                ex[r + 1] = x < y ? y : x;
+            }
+            // The pair across lanes: (lane, R-1) and (lane+1, 0).
+            const unsigned long long nx = ((unsigned long long)(unsigned)__shfl_down((int)(ex[0] >> 32), 1) << 32) |
+                                          (unsigned)__shfl_down((int)(unsigned)ex[0], 1);
+            const unsigned long long px = ((unsigned long long)(unsigned)__shfl_up((int)(ex[R - 1] >> 32), 1) << 32) |
+                                          (unsigned)__shfl_up((int)(unsigned)ex[R - 1], 1);
+            const bool lsw = lane < kWave - 1 && ex[R - 1] > nx;
+            const bool rsw = lane > 0 && px > ex[0];
+            ex[R - 1] = lsw ? nx : ex[R - 1];
+            ex[0] = rsw ? px : ex[0];
+            swapped |= lsw;
+            if (__ballot(swapped) == 0) break;
+        }
+        float kx[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            v[r] = (unsigned)ex[r] & 0x7ffu;
+            kx[r] = ex[r] == ~0ull ? __builtin_inff() : __uint_as_float((unsigned)(ex[r] >> 32));
+        }
 
         // ord row: this lane's R consecutive positions, 16 bytes at a time.
         uint16_t *orow = a.ord + ((long long)slot * n + i) * a.rs;
@@ -303,29 +430,27 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
             const long long e0 = (long long)lane * R + r0;
             if (e0 < a.rs) {
                 uint4 w;
-                w.x = (unsigned)(v[r0 + 0] & 0xffff) | ((unsigned)(v[r0 + 1] & 0xffff) << 16);
-                w.y = (unsigned)(v[r0 + 2] & 0xffff) | ((unsigned)(v[r0 + 3] & 0xffff) << 16);
-                w.z = (unsigned)(v[r0 + 4] & 0xffff) | ((unsigned)(v[r0 + 5] & 0xffff) << 16);
-                w.w = (unsigned)(v[r0 + 6] & 0xffff) | ((unsigned)(v[r0 + 7] & 0xffff) << 16);
+                w.x = (v[r0 + 0] & 0x7ffu) | ((v[r0 + 1] & 0x7ffu) << 16);
+                w.y = (v[r0 + 2] & 0x7ffu) | ((v[r0 + 3] & 0x7ffu) << 16);
+                w.z = (v[r0 + 4] & 0x7ffu) | ((v[r0 + 5] & 0x7ffu) << 16);
+                w.w = (v[r0 + 6] & 0x7ffu) | ((v[r0 + 7] & 0x7ffu) << 16);
                 *reinterpret_cast<uint4 *>(orow + e0) = w;
             }
         }
 
         // Upper-bound rank of position e: the first end of an equal-key run
-        // at or after e (a run ends where the next key differs; the padding
-        // keys 0xffffffff differ from every finite key).
+        // at or after e (a run ends where the next exact key differs; the
+        // padding's +inf differs from every finite key).
         constexpr unsigned kBig = 0x7fffffffu;
-        const unsigned next0 = (unsigned)__shfl_down((int)(unsigned)(v[0] >> 32), 1);
-        unsigned s[R];
+        const float next0 = __shfl_down(kx[0], 1);
+        unsigned sfx[R];
         unsigned m = kBig;
 #pragma unroll
         for (int r = R - 1; r >= 0; --r) {
-            const unsigned kr = (unsigned)(v[r] >> 32);
-            const unsigned kn = (r == R - 1) ? (lane == kWave - 1 ? 0xffffffffu : next0)
-                                             : (unsigned)(v[r + 1] >> 32);
-            const unsigned end = kn != kr ? (unsigned)(lane * R + r + 1) : kBig;
-            m = min(m, end);
-            s[r] = m;
+            const float kn = (r == R - 1) ? (lane == kWave - 1 ? __builtin_inff() : next0) : kx[r + 1];
+            const bool end = (kn != kx[r]) || (r == R - 1 && lane == kWave - 1);
+            m = end ? min(m, (unsigned)(lane * R + r + 1)) : m;
+            sfx[r] = m;
         }
         unsigned t = m;
 #pragma unroll
@@ -338,7 +463,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int e = lane * R + r;
-            if (e < n) st[(unsigned)(v[r] & 0xffff)] = (uint16_t)min(s[r], after);
+            if (e < n) st[v[r]] = (uint16_t)min(sfx[r], after);
         }
         ksg::wave_sync();
         uint16_t *rrow = a.rank + ((long long)slot * n + i) * a.rs;
