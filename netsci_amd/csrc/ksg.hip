@@ -390,6 +390,31 @@ __global__ __launch_bounds__(1024) void slot_scan_kernel(const int *used, int N,
     if (threadIdx.x == 0) *count = carry;
 }
 
+// span[s] = max over dimensions d of (max - min) of coordinate d of node
+// node_of[s] over the window's n frames: an upper bound of every marginal
+// distance of the node (rounding of the differences is monotone). One wave
+// per node.
+__global__ __launch_bounds__(64) void span_kernel(const float *X, long long ld, long long dstride,
+                                                  int d, int n, const int *node_of, float *span) {
+    const float *x = X + (long long)node_of[blockIdx.x] * ld;
+    float best = 0.0f;
+    for (int t = 0; t < d; ++t) {
+        float lo = __builtin_inff(), hi = -__builtin_inff();
+        for (int f = threadIdx.x; f < n; f += 64) {
+            const float v = x[t * dstride + f];
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            lo = fminf(lo, __shfl_xor(lo, off));
+            hi = fmaxf(hi, __shfl_xor(hi, off));
+        }
+        best = fmaxf(best, hi - lo);
+    }
+    if (threadIdx.x == 0) span[blockIdx.x] = best;
+}
+
 // psi terms of every frame summed sequentially in frame order, then the MI
 // (mutual_information_cpu.cpp:110-118). A wave owns 64 consecutive pairs,
 // one lane each: tiles of 64 pairs x 64 frames are read row by row
@@ -477,9 +502,10 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     NSX_HIP(hipStreamSynchronize(stream), "ab range sync");
     NSX_REQUIRE(h_mm[0] >= 0, "ksg: negative node index in ab");
     const int N = h_mm[1] + 1;
-    int *map = (int *)workspace(kWsListMap, ((size_t)3 * N + 16) * sizeof(int));
+    int *map = (int *)workspace(kWsListMap, ((size_t)4 * N + 16) * sizeof(int));
     if (!map) return kHipError;
     int *used = map, *slot_of = map + N, *node_of = map + 2 * N, *d_count = map + 3 * N;
+    float *span = (float *)(map + 3 * N + 16);  // per table slot
     rc = fill_u32(used, 0u, (size_t)N, stream);
     if (rc) return rc;
     hipLaunchKernelGGL(mark_nodes_kernel, dim3((unsigned)rblocks), dim3(256), 0, stream, pb.ab,
@@ -583,6 +609,9 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
                            dim3(ksgw::kWave * ksgw::kBuildWaves),
                            ksgw::build_lds_bytes(pb.d, n, rs), stream, ba);
         NSX_HIP(hipGetLastError(), "list build launch");
+        hipLaunchKernelGGL(span_kernel, dim3((unsigned)U), dim3(64), 0, stream, X, pb.ld,
+                           pb.dstride, pb.d, n, node_of, span);
+        NSX_HIP(hipGetLastError(), "span launch");
         if (b1) (void)hipEventRecord(b1, stream);
         marks.push_back({0, b0, b1});
         if (w == 0) marks.push_back({0, ev_start, b0});  // the node map
@@ -629,6 +658,9 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
             wa.nbuf = nbuf;
             wa.batch = g_walk_batch;
             wa.checkpoint = 8;
+            wa.span = span;
+            wa.id_bits = ksgw::heap_id_bits(n);
+            wa.code_shift = ksgw::heap_code_shift(n);
             const long long grid = std::max<long long>(1, std::min<long long>(Pc, grid_max));
             rc = fill_u32(tie_count, 0u, 1, stream);
             if (rc) return rc;

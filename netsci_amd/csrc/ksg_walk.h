@@ -103,6 +103,9 @@ struct WalkArgs {
     int nbuf;                   // pair buffers per workgroup (LDS ring)
     int batch;                  // lanes that wait before a refill round
     int checkpoint;             // loop passes between workgroup checkpoints
+    const float *span;          // per table slot: max over d of (max - min) of coordinate d
+    int id_bits;                // frame bits of a heap entry (2^id_bits >= n)
+    int code_shift;             // float-bit units per distance code step: 2^code_shift
 };
 
 // Frame-order psi sums of the packed counts (one lane per pair).
@@ -126,8 +129,19 @@ inline size_t build_lds_bytes(int d, int n, long long rs) {
     return (((size_t)d * n * 4 + 15) & ~(size_t)15) + (size_t)kBuildWaves * rs * 2;
 }
 inline size_t walk_lds_bytes(int d, int n, int nbuf) {
-    return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 4 * nbuf) * sizeof(int);
+    return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 5 * nbuf) * sizeof(int);
 }
+
+// Heap entries of the walk: (distance code << id_bits) | frame in 32 bits,
+// id_bits the frame bits for n frames. The code covers the 2^27 float-bit
+// values (16 binades) below the pair's coordinate span at 2^code_shift per
+// step: id_bits + (32 - id_bits) code bits - code_shift = 27.
+inline int heap_id_bits(int n) {
+    int b = 1;
+    while ((1 << b) < n) ++b;
+    return b;
+}
+inline int heap_code_shift(int n) { return heap_id_bits(n) > 5 ? heap_id_bits(n) - 5 : 0; }
 
 // Sort elements per wave-lane for n frames: 8, 16 or 32 for one wave per
 // row; 64 and 128 mean 2 and 4 waves per row (32 each).
@@ -584,19 +598,30 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
     }
 }
 
-// Insert (c, j) into the ascending h[0..K] without branches: every slot
-// is two compares, a med3 and two selects. c >= h[K] (+inf for a rejected
-// candidate) leaves everything unchanged.
+// med3 of unsigned 32-bit values (v_med3_u32; the compiler forms it only
+// for constant bounds).
+__device__ __forceinline__ unsigned umed3(unsigned x, unsigned y, unsigned z) {
+    unsigned r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(y), "v"(z));
+    return r;
+}
+
+// Insert c into the ascending h[0..K]: one med3 per slot. c >= h[K]
+// (0xffffffff for a rejected candidate) leaves everything unchanged.
 template <int K>
-__device__ __forceinline__ void insert_flat(float (&h)[K + 1], int (&id)[K + 1], float c, int j) {
+__device__ __forceinline__ void insert_packed(unsigned (&h)[K + 1], unsigned c) {
 #pragma unroll
-    for (int m = K; m >= 1; --m) {
-        const int t = c < h[m] ? j : id[m];
-        id[m] = c < h[m - 1] ? id[m - 1] : t;
-        h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
-    }
-    id[0] = c < h[0] ? j : id[0];
-    h[0] = __builtin_amdgcn_fmed3f(c, h[0], -__builtin_inff());  // min of non-NaN values
+    for (int m = K; m >= 1; --m) h[m] = umed3(h[m - 1], c, h[m]);
+    h[0] = min(h[0], c);
+}
+
+// Distance code of a key for a pair whose code window starts at float bits
+// `base`: monotone non-decreasing in the key, so code(x) < code(y) implies
+// x < y. Keys never exceed the pair's span (both coordinates lie within
+// each dimension's [min, max], and rounding is monotone), so the code fits
+// the 32 - id_bits code bits.
+__device__ __forceinline__ unsigned key_code(float key, unsigned base, int shift) {
+    return __builtin_elementwise_sub_sat(__float_as_uint(key), base) >> shift;
 }
 
 // (D, j) as one signed 64-bit value ordered like the list's (D, j) order:
@@ -645,7 +670,10 @@ __device__ __forceinline__ int ring_slot(int s, int nbuf) {
 }
 
 // Stage local pair s of this workgroup into ring buffer s % nbuf: both rows'
-// coordinates as per-frame records, and the two nodes' table slots.
+// coordinates as per-frame records, the two nodes' table slots and the
+// pair's code base: the larger span's float bits minus the code window's
+// width less one (at least 0), so that every key's code fits 32 - id_bits
+// bits.
 template <int D>
 __device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *meta, long long s,
                                            int n) {
@@ -661,8 +689,12 @@ __device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *
         rec[2 * D * f + D + d] = xb[d * a.dstride + f];
     }
     if (threadIdx.x == 0) {
-        meta[2 * b] = a.slot_of[na];
-        meta[2 * b + 1] = a.slot_of[nb];
+        const int sa = a.slot_of[na], sb = a.slot_of[nb];
+        meta[3 * b] = sa;
+        meta[3 * b + 1] = sb;
+        const long long window = 1LL << (32 - a.id_bits + a.code_shift);
+        const long long top = (long long)__float_as_uint(fmaxf(a.span[sa], a.span[sb]));
+        meta[3 * b + 2] = (int)(unsigned)(top - (window - 1) > 0 ? top - (window - 1) : 0);
     }
 }
 
@@ -682,11 +714,13 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     const int rec_floats = 2 * D * n;
     int *ctrl = reinterpret_cast<int *>(lds + nbuf * rec_floats);
     // ctrl[0] next query, [1] staged pairs, [2] retired pairs, [3] pairs to
-    // stage now, [4..4+nbuf) done queries per buffer, then 2 table slots per
-    // buffer, then the pairs to stage.
+    // stage now, [8..8+nbuf) done queries per buffer, then per buffer the 2
+    // table slots and the code base, then the pairs to stage.
     int *done = ctrl + 8;
     int *meta = done + nbuf;
-    int *to_stage = meta + 2 * nbuf;
+    int *to_stage = meta + 3 * nbuf;
+    const int idb = a.id_bits, shift = a.code_shift;
+    const unsigned id_mask = (1u << idb) - 1u;
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int wpb = blockDim.x / kWave;
@@ -718,11 +752,17 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     int rec = 0;  // LDS float offset of the lane's pair records (an offset, not a
                   // pointer, so every access stays an LDS instruction)
     long long offA = 0, offB = 0;  // table row offsets of (node, frame i)
+    unsigned base = 0;             // the pair's code base
     // Last walked entry of each list as lex(D, j); -1: none yet (its key
     // reads as NaN, so the stop test below is false).
     long long lastA = -1, lastB = -1;
-    float qa[D], qb[D], h[K + 1];
-    int id[K + 1];
+    float qa[D], qb[D];
+    // The k+1 smallest (code(z) << idb | j) of the frames evaluated, in
+    // ascending order. Codes are monotone in z, so when the k-th and
+    // (k+1)-th codes differ the first k entries are the k nearest frames
+    // exactly; equal codes (a tie of z, or two z too close for the code)
+    // go to the tie list.
+    unsigned h[K + 1];
     // List blocks: cur holds entries [pos, pos + 8) of each list, nxt the 8
     // after them (each list's pos is a multiple of 8 at every body start,
     // until the list is exhausted).
@@ -735,10 +775,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
 #pragma unroll
     for (int d = 0; d < D; ++d) qa[d] = qb[d] = 0.0f;
 #pragma unroll
-    for (int m = 0; m <= K; ++m) {
-        h[m] = __builtin_inff();
-        id[m] = 0;
-    }
+    for (int m = 0; m <= K; ++m) h[m] = 0xffffffffu;
 
     for (int pass = 0;; ++pass) {
         if (pass % a.checkpoint == 0) {
@@ -776,18 +813,19 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             if (pend) a.cnt[pend_at] = (pend_x - 1u) | ((pend_y - 1u) << 16);
             pend = false;
             // (1) eps of the finished walks.
-            const bool tie = res && (h[K - 1] == h[K]);
+            const bool tie = res && ((h[K - 1] ^ h[K]) >> idb) == 0u;
             int jx = 0, jy = 0;
             if (res && !tie) {
                 float ex = -1.0f, ey = -1.0f;
 #pragma unroll
                 for (int m = 0; m < K; ++m) {
+                    const int j = (int)(h[m] & id_mask);
                     float va[D], vb[D];
-                    fetch<D>(lds + rec, id[m], va, vb);
+                    fetch<D>(lds + rec, j, va, vb);
                     const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
-                    jx = da > ex ? id[m] : jx;
+                    jx = da > ex ? j : jx;
                     ex = fmaxf(ex, da);
-                    jy = db > ey ? id[m] : jy;
+                    jy = db > ey ? j : jy;
                     ey = fmaxf(ey, db);
                 }
             }
@@ -888,14 +926,12 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 rec = b * rec_floats;
                 fetch<D>(lds + rec, i, qa, qb);
 #pragma unroll
-                for (int m = 0; m <= K; ++m) {
-                    h[m] = __builtin_inff();
-                    id[m] = 0;
-                }
+                for (int m = 0; m <= K; ++m) h[m] = 0xffffffffu;
                 posA = posB = 0;
                 lastA = lastB = -1;
-                offA = ((long long)meta[2 * b] * n + i) * rs;
-                offB = ((long long)meta[2 * b + 1] * n + i) * rs;
+                offA = ((long long)meta[3 * b] * n + i) * rs;
+                offB = ((long long)meta[3 * b + 1] * n + i) * rs;
+                base = (unsigned)meta[3 * b + 2];
                 curA = load_block(a.ord + offA);
                 curB = load_block(a.ord + offB);
                 if (rs > 8) {
@@ -917,12 +953,13 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         // walks about 40% fewer entries than taking one of each per step.
         bool useA;
         {
-            const float hk = h[K];
+            // h[K]'s distance, to within its code step.
+            const float hk = __uint_as_float(((h[K] >> idb) << shift) + base);
             if (posA >= n) {
                 useA = false;
             } else if (posB >= n) {
                 useA = true;
-            } else if (posA < 8 || posB < 8 || !(hk < __builtin_inff())) {
+            } else if (posA < 8 || posB < 8 || h[K] == 0xffffffffu) {
                 useA = posA <= posB;
             } else {
                 // An estimate only: the hardware reciprocal is precise enough.
@@ -960,11 +997,14 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 // Already evaluated if the other list has walked it.
                 const bool seen = lex(keyO, j) <= lastO;
                 lastL = lex(keyL, j);
-                const float z = (seen || j == i) ? __builtin_inff() : fmaxf(da, db);
-                insert_flat<K>(h, id, z, j);
-                // Done when the stopping rule holds or both lists are
-                // exhausted; this body ends too when this list is.
-                const bool fin = (fmaxf(keyL, lex_key(lastO)) > h[K]) | ((posL >= n) & (posO >= n));
+                const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
+                insert_packed<K>(h, (seen || j == i) ? 0xffffffffu : c);
+                // Done when the stopping rule holds (in codes: a larger code
+                // means a larger distance; a missing last key reads as NaN,
+                // which fmaxf ignores) or both lists are exhausted; this
+                // body ends too when this list is.
+                const unsigned stop = key_code(fmaxf(keyL, lex_key(lastO)), base, shift) << idb;
+                const bool fin = (stop > h[K]) | ((posL >= n) & (posO >= n));
                 on = !fin & (posL < n);
                 state = fin ? 0 : state;
                 res = res | fin;
@@ -998,10 +1038,13 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     }
 }
 
-// Resolve the tied queries the walk deferred: one wave per entry rebuilds
-// the query's joint distance row, selects positions 0..k in libstdc++
-// std::sort order (mutual_information_cpu.cpp:48-77), takes eps over
-// positions 1..k and stores the counts from the rank tables.
+// Resolve the queries the walk deferred (equal k-th and (k+1)-th heap
+// codes): one wave per entry rebuilds the query's joint distance row. When
+// the k-th and (k+1)-th smallest distances over the other frames differ
+// (the codes only collided), the k nearest frames are those with distance
+// <= the k-th, as in the walk. Otherwise the row's positions 0..k are
+// selected in libstdc++ std::sort order (mutual_information_cpu.cpp:48-77)
+// and eps taken over positions 1..k. The counts come from the rank tables.
 template <int D, int K>
 __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
     // Each wave's row of n records lives in LDS: the selection's partition
@@ -1027,6 +1070,11 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
             qa[d] = xa[d * a.dstride + i];
             qb[d] = xb[d * a.dstride + i];
         }
+        // The row (self included, as the reference sorts it) and each
+        // lane's k+1 smallest distances over the other frames.
+        float h[K + 1];
+#pragma unroll
+        for (int m = 0; m <= K; ++m) h[m] = __builtin_inff();
         for (int j = lane; j < n; j += kWave) {
             float va[D], vb[D];
 #pragma unroll
@@ -1038,12 +1086,56 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
             r.key = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
             r.idx = j;
             scratch[j] = r;
+            const float c = j == i ? __builtin_inff() : r.key;
+#pragma unroll
+            for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+            h[0] = fminf(h[0], c);
+        }
+        // Merge across the wave: afterwards every lane holds the row's k+1
+        // smallest.
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            float o[K + 1];
+#pragma unroll
+            for (int m = 0; m <= K; ++m) o[m] = __shfl_xor(h[m], off);
+#pragma unroll
+            for (int t = 0; t <= K; ++t) {
+#pragma unroll
+                for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], o[t], h[m]);
+                h[0] = fminf(h[0], o[t]);
+            }
         }
         ksg::wave_sync();
-        if (lane == 0) {
+        int jx = 0, jy = 0;
+        if (h[K - 1] < h[K]) {
+            float ex = -1.0f, ey = -1.0f;
+            for (int j = lane; j < n; j += kWave) {
+                if (j == i || !(scratch[j].key <= h[K - 1])) continue;
+                float va[D], vb[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    va[d] = xa[d * a.dstride + j];
+                    vb[d] = xb[d * a.dstride + j];
+                }
+                const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+                jx = da > ex ? j : jx;
+                ex = fmaxf(ex, da);
+                jy = db > ey ? j : jy;
+                ey = fmaxf(ey, db);
+            }
+            // Any frame of the largest marginal distance gives the same rank.
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const float oex = __shfl_xor(ex, off), oey = __shfl_xor(ey, off);
+                const int ojx = __shfl_xor(jx, off), ojy = __shfl_xor(jy, off);
+                jx = oex > ex ? ojx : jx;
+                ex = fmaxf(ex, oex);
+                jy = oey > ey ? ojy : jy;
+                ey = fmaxf(ey, oey);
+            }
+        } else if (lane == 0) {
             sort_prefix(scratch, n, K + 1);
             float ex = -1.0f, ey = -1.0f;
-            int jx = 0, jy = 0;
             for (int m = 1; m <= K; ++m) {
                 const int j = scratch[m].idx;
                 float va[D], vb[D];
@@ -1058,6 +1150,8 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
                 jy = db > ey ? j : jy;
                 ey = fmaxf(ey, db);
             }
+        }
+        if (lane == 0) {
             const long long offA = ((long long)a.slot_of[na] * n + i) * a.rs;
             const long long offB = ((long long)a.slot_of[nb] * n + i) * a.rs;
             const unsigned rx = a.rank[offA + jx], ry = a.rank[offB + jy];

@@ -7,10 +7,16 @@ facts: a frame already walked in the other list is recognised by a
 lexicographic (distance, frame) compare; once max(last walked keys) exceeds
 the (k+1)-th smallest joint distance no unwalked frame can enter the k+1
 nearest; and n_x + 1 = rank_a[i][j*] for the selected frame j* of largest
-marginal distance. This test runs the same steps in numpy (one list block of
-8 entries at a time, chosen by the kernel's remaining-entries estimate) and
-requires the oracle's per-frame counts (mutual_information_cpu.cpp:22-109),
-ties and duplicate frames included. It pins the algorithm, not the kernel;
+marginal distance. The kernel keeps its k+1 nearest as 32-bit entries
+(distance code << id_bits | frame), the code a monotone, lossy function of
+the distance's float bits in a window below the pair's coordinate span;
+walks whose k-th and (k+1)-th codes are equal go to the tie resolution,
+which separates code collisions from exact ties. This test runs the same
+steps in numpy (one list block of 8 entries at a time, chosen by the
+kernel's remaining-entries estimate) and requires the oracle's per-frame
+counts (mutual_information_cpu.cpp:22-109), ties and duplicate frames
+included, with the kernel's code step and with a far coarser one that makes
+most walks collide. It pins the algorithm, not the kernel;
 tests/test_gpu_ksg.py pins the kernel.
 """
 import numpy as np
@@ -39,31 +45,55 @@ def _tables(c, n):
     return np.array(ords), np.array(ranks)
 
 
-def _walk_counts(xa, xb, n, d, k, bug=None):
+def _bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def _id_bits(n):
+    b = 1
+    while (1 << b) < n:
+        b += 1
+    return b
+
+
+def _walk_counts(xa, xb, n, d, k, bug=None, shift=None):
     ca, cb = xa.reshape(d, n), xb.reshape(d, n)
     oa, ra = _tables(ca, n)
     ob, rb = _tables(cb, n)
+    idb = _id_bits(n)
+    if shift is None:
+        shift = max(idb - 5, 0)
+    span = max(float(np.max(np.float32(r.max()) - np.float32(r.min()))) for r in (*ca, *cb))
+    window = 1 << (32 - idb + shift)
+    base = max(_bits(span) - (window - 1), 0)
+    mask = (1 << idb) - 1
+    sentinel = 0xFFFFFFFF
+
+    def code(key):
+        return max(_bits(key) - base, 0) >> shift
+
     nx = np.zeros(n, np.int64)
     ny = np.zeros(n, np.int64)
     for i in range(n):
         DA, DB = _marg(ca, i), _marg(cb, i)
-        h = [np.float32(np.inf)] * (k + 1)
-        ids = [0] * (k + 1)
+        h = [sentinel] * (k + 1)
         last = {"a": (-1.0, -1), "b": (-1.0, -1)}  # (key, frame); -1: none yet
         pos = {"a": 0, "b": 0}
         done = False
         while not done:
             # list choice per 8-entry block, as the kernel
+            hk = np.uint32(((h[k] >> idb) << shift) + base).view(np.float32) \
+                if h[k] != sentinel else None
             if pos["a"] >= n:
                 L = "b"
             elif pos["b"] >= n:
                 L = "a"
-            elif pos["a"] < 8 or pos["b"] < 8 or not np.isfinite(h[k]):
+            elif pos["a"] < 8 or pos["b"] < 8 or hk is None:
                 L = "a" if pos["a"] <= pos["b"] else "b"
             else:
                 with np.errstate(over="ignore"):
-                    ea = pos["a"] * ((h[k] / max(last["a"][0], 1e-30)) ** d - 1)
-                    eb = pos["b"] * ((h[k] / max(last["b"][0], 1e-30)) ** d - 1)
+                    ea = pos["a"] * ((hk / max(last["a"][0], 1e-30)) ** d - 1)
+                    eb = pos["b"] * ((hk / max(last["b"][0], 1e-30)) ** d - 1)
                 L = "a" if ea <= eb else "b"
             O = "b" if L == "a" else "a"
             order = oa if L == "a" else ob
@@ -80,21 +110,23 @@ def _walk_counts(xa, xb, n, d, k, bug=None):
                     seen = kO <= last[O][0] if last[O][1] >= 0 else False
                 last[L] = (kL, j)
                 if not seen and j != i:
-                    z = max(DA[j], DB[j])
-                    if z < h[k]:
-                        m = k
-                        while m > 0 and z < h[m - 1]:
-                            h[m], ids[m] = h[m - 1], ids[m - 1]
-                            m -= 1
-                        h[m], ids[m] = z, j
+                    c = (code(max(DA[j], DB[j])) << idb) | int(j)
+                    h = sorted(h + [c])[:k + 1]
                 keys = [v[0] for v in last.values() if v[1] >= 0]
-                if max(keys) > h[k] or (pos["a"] >= n and pos["b"] >= n):
+                if (code(max(keys)) << idb) > h[k] or (pos["a"] >= n and pos["b"] >= n):
                     done = True
                     break
-        if h[k - 1] == h[k]:  # exact tie at the k-th distance: std::sort order
-            sel = oracle.argsort_like_reference(np.maximum(DA, DB))[1:k + 1]
+        if (h[k - 1] ^ h[k]) >> idb == 0 and bug != "collisions_as_distinct":
+            # Tie resolution: exact k-th / (k+1)-th distances over the other
+            # frames; equal ones take the std::sort order.
+            z = np.maximum(DA, DB)
+            zo = np.sort(np.delete(z, i))
+            if zo[k - 1] < zo[k]:
+                sel = np.array([j for j in range(n) if j != i and z[j] <= zo[k - 1]])
+            else:
+                sel = oracle.argsort_like_reference(z)[1:k + 1]
         else:
-            sel = np.array(ids[:k])
+            sel = np.array([v & mask for v in h[:k]])
         jx = sel[int(np.argmax(DA[sel]))]
         jy = sel[int(np.argmax(DB[sel]))]
         nx[i] = ra[i, jx] - 1
@@ -120,32 +152,55 @@ def _case(kind, d, n, rng):
     return xa.astype(np.float32), xb.astype(np.float32)
 
 
+@pytest.mark.parametrize("coarse", [False, True], ids=["code", "coarse_code"])
 @pytest.mark.parametrize("kind", ["gauss", "grid", "dups", "mixed"])
 @pytest.mark.parametrize("d", [1, 2, 3])
-def test_walk_counts_equal_oracle(kind, d):
+def test_walk_counts_equal_oracle(kind, d, coarse):
     rng = np.random.default_rng(hash((kind, d)) % 2**32)
     for _ in range(3):
         n = int(rng.integers(9, 90))
         k = int(rng.integers(1, min(8, n - 1) + 1))
         xa, xb = _case(kind, d, n, rng)
-        nx, ny = _walk_counts(xa, xb, n, d, k)
+        # coarse: 2^19 float-bit units per code step, 1/16 binade
+        nx, ny = _walk_counts(xa, xb, n, d, k, shift=19 if coarse else None)
         ox, oy, _, _ = oracle.ksg_counts(xa, xb, k, n, d)
         np.testing.assert_array_equal(nx, ox)
         np.testing.assert_array_equal(ny, oy)
 
 
-@pytest.mark.parametrize("bug,kind", [("no_seen", "dups"), ("key_only_seen", "grid")])
+def test_code_window_holds_every_key():
+    """No key's code exceeds the code bits (it would spill into the frame
+    bits): keys are at most the pair's span, including the extreme case of
+    coordinates at both ends of the float range's finite half."""
+    rng = np.random.default_rng(5)
+    for n in (9, 100, 2000, 2048, 2049, 8192):
+        idb = _id_bits(n)
+        shift = max(idb - 5, 0)
+        for scale in (1e-30, 1.0, 1e30):
+            x = (rng.standard_normal(64) * scale).astype(np.float32)
+            span = np.float32(x.max()) - np.float32(x.min())
+            base = max(_bits(span) - ((1 << (32 - idb + shift)) - 1), 0)
+            keys = np.abs(x[:, None] - x[None, :]).astype(np.float32).ravel()
+            codes = np.maximum(keys.view(np.uint32).astype(np.int64) - base, 0) >> shift
+            assert codes.max() < (1 << (32 - idb))
+
+
+@pytest.mark.parametrize("bug,kind", [("no_seen", "dups"), ("key_only_seen", "grid"),
+                                      ("collisions_as_distinct", "gauss")])
 def test_planted_mistakes_are_caught(bug, kind):
     """A restatement that inserts a frame walked in both lists twice (no
-    seen test), or treats every frame whose key equals the other list's last
-    key as walked (no frame tie-break), must disagree with the oracle on
-    duplicate-frame or tie-heavy inputs, so the comparison above can fail."""
+    seen test), treats every frame whose key equals the other list's last
+    key as walked (no frame tie-break), or takes the k entries of a walk
+    whose k-th and (k+1)-th codes collide as the k nearest, must disagree
+    with the oracle on duplicate-frame, tie-heavy or coarse-code inputs, so
+    the comparison above can fail."""
     rng = np.random.default_rng(11)
     mismatched = 0
     for _ in range(4):
         d, n, k = 2, 60, 4
         xa, xb = _case(kind, d, n, rng)
-        nx, ny = _walk_counts(xa, xb, n, d, k, bug=bug)
+        nx, ny = _walk_counts(xa, xb, n, d, k, bug=bug,
+                              shift=19 if bug == "collisions_as_distinct" else None)
         ox, oy, _, _ = oracle.ksg_counts(xa, xb, k, n, d)
         mismatched += int(not (np.array_equal(nx, ox) and np.array_equal(ny, oy)))
     assert mismatched > 0
