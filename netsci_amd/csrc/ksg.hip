@@ -321,11 +321,11 @@ ksgw::WalkFn walk_for(int d, int k) {
     }
 }
 
-ksgw::TieFn tie_for(int d, int k) {
+ksgw::TieFn tie_for(int d, int k, bool overflow) {
     switch (d) {
-        case 1: return ksgw::tie_d1(k);
-        case 2: return ksgw::tie_d2(k);
-        case 3: return ksgw::tie_d3(k);
+        case 1: return ksgw::tie_d1(k, overflow);
+        case 2: return ksgw::tie_d2(k, overflow);
+        case 3: return ksgw::tie_d3(k, overflow);
         default: return nullptr;
     }
 }
@@ -477,8 +477,9 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
         return kOk;
     }
     const ksgw::WalkFn walk = walk_for(pb.d, pb.k);
-    const ksgw::TieFn tiek = tie_for(pb.d, pb.k);
-    NSX_REQUIRE(walk && tiek, "ksg: no list-path kernel for (d, k)");
+    const ksgw::TieFn tiek = tie_for(pb.d, pb.k, false);
+    const ksgw::TieFn tieo = tie_for(pb.d, pb.k, true);
+    NSX_REQUIRE(walk && tiek && tieo, "ksg: no list-path kernel for (d, k)");
     const long long m2 = 2 * pb.P;
     for (int t = 0; t < 4; ++t) g_last_phase_ms[t] = 0;
     hipEvent_t ev_start = g_phase_events.get(0);
@@ -552,8 +553,8 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     // pair chunks), so local (pair, frame) indices stay below 2^30.
     const long long chunk = std::max<long long>(
         1, std::min<long long>(pb.P, (1LL << 32) / ((long long)pb.nmax * (long long)sizeof(unsigned))));
-    // Tie list: up to 1/32 of a chunk's queries (ties beyond it are
-    // resolved inside the walk).
+    // Tie list: up to 1/32 of a chunk's queries (ties beyond it are marked
+    // in the counts and resolved by tie_overflow_kernel).
     const unsigned tie_cap =
         (unsigned)std::max<long long>(1024, chunk * (long long)pb.nmax / 32);
     const long long tie_waves = (long long)cu_count() * 8;
@@ -561,12 +562,10 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     // by less than nmax.
     const size_t cnt_words = (size_t)chunk * pb.nmax + pb.nmax;
     unsigned *cnt = (unsigned *)workspace(kWsListCnt, (cnt_words + tie_cap) * sizeof(unsigned));
-    KeyIdx *scratch =
-        (KeyIdx *)workspace(kWsTieScratch, (size_t)grid_max * wpb * pb.nmax * sizeof(KeyIdx));
     unsigned long long *stats = (unsigned long long *)(mm + 8);
     unsigned *tie_count = (unsigned *)(mm + 16);
     unsigned *bad = (unsigned *)(mm + 4);
-    if (!cnt || !scratch) return kHipError;
+    if (!cnt) return kHipError;
     unsigned *tie_list = cnt + cnt_words;
     rc = fill_u32(stats, 0u, 4, stream);
     if (rc) return rc;
@@ -649,7 +648,6 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
             wa.rank = rank;
             wa.rs = rs;
             wa.slot_of = slot_of;
-            wa.scratch = scratch;
             wa.cnt = cnt;
             wa.stats = stats;
             wa.tie_list = tie_list;
@@ -676,6 +674,10 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
                                dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * n * sizeof(KeyIdx),
                                stream, wa);
             NSX_HIP(hipGetLastError(), "tie launch");
+            hipLaunchKernelGGL(tieo, dim3((unsigned)(tie_waves / tie_wpb)),
+                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * n * sizeof(KeyIdx),
+                               stream, wa);
+            NSX_HIP(hipGetLastError(), "tie overflow launch");
             if (e2) (void)hipEventRecord(e2, stream);
             ksgw::SumArgs sa;
             sa.cnt = cnt;

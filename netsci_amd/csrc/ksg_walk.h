@@ -84,6 +84,10 @@ struct BuildArgs {
     unsigned *bad;      // set to 1 when a coordinate is not finite
 };
 
+// Packed counts of a tied query the tie list had no room for: the counts
+// of a real query are below n - 1 <= 8191 each.
+constexpr unsigned kTieMark = 0xffffffffu;
+
 struct WalkArgs {
     const float *X;
     long long ld, dstride;
@@ -94,11 +98,11 @@ struct WalkArgs {
     const uint16_t *ord, *rank;
     long long rs;
     const int *slot_of;         // node -> table slot
-    KeyIdx *scratch;            // per wave: n records (tie rows)
     unsigned *cnt;              // [P][n] packed counts n_x | n_y << 16
     unsigned long long *stats;  // [0] tie frames, [1] list entries walked (both lists)
     unsigned *tie_list;         // tied queries (local pair * n + frame), resolved later
-    unsigned *tie_count;        // entries appended (may exceed tie_cap)
+    unsigned *tie_count;        // entries appended (may exceed tie_cap; the rest are
+                                // marked kTieMark in cnt)
     unsigned tie_cap;
     int nbuf;                   // pair buffers per workgroup (LDS ring)
     int batch;                  // lanes that wait before a refill round
@@ -157,9 +161,10 @@ BuildFn build_d3(int width);
 WalkFn walk_d1(int k);
 WalkFn walk_d2(int k);
 WalkFn walk_d3(int k);
-TieFn tie_d1(int k);
-TieFn tie_d2(int k);
-TieFn tie_d3(int k);
+// overflow: tie_overflow_kernel instead of tie_kernel.
+TieFn tie_d1(int k, bool overflow);
+TieFn tie_d2(int k, bool overflow);
+TieFn tie_d3(int k, bool overflow);
 
 #if defined(__HIPCC__)
 
@@ -653,6 +658,11 @@ __device__ __forceinline__ void fetch(const float *rec, int j, float (&va)[D], f
     }
 }
 
+// Component-wise select (a select of whole vectors can go through memory).
+__device__ __forceinline__ uint4 sel4(bool c, uint4 x, uint4 y) {
+    return make_uint4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
+}
+
 __device__ __forceinline__ uint4 load_block(const uint16_t *p) {
     return *reinterpret_cast<const uint4 *>(p);
 }
@@ -722,9 +732,6 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     const int idb = a.id_bits, shift = a.code_shift;
     const unsigned id_mask = (1u << idb) - 1u;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int wpb = blockDim.x / kWave;
-    KeyIdx *scratch = a.scratch + ((long long)blockIdx.x * wpb + wib) * n;
     const long long rs = a.rs;
     const long long G = gridDim.x;
     const long long npairs =
@@ -829,71 +836,30 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                     ey = fmaxf(ey, db);
                 }
             }
-            // Ties go to the tie list (resolved by tie_kernel after the walk);
-            // only when it is full are they resolved here.
+            // Ties go to the tie list, resolved by tie_kernel after the walk;
+            // those beyond its capacity are marked in cnt for
+            // tie_overflow_kernel.
             unsigned long long tm = __ballot(tie);
             ties += (unsigned)__popcll(tm);
             bool deferred = false;
             if (tm) {
                 const int leader = __builtin_amdgcn_readfirstlane(__ffsll((long long)tm) - 1);
-                unsigned base = 0;
-                if (lane == leader) base = atomicAdd(a.tie_count, (unsigned)__popcll(tm));
-                base = __shfl(base, leader);
-                const unsigned slot = base + __builtin_amdgcn_mbcnt_hi(
-                                                 (unsigned)(tm >> 32),
-                                                 __builtin_amdgcn_mbcnt_lo((unsigned)tm, 0u));
+                unsigned tbase = 0;
+                if (lane == leader) tbase = atomicAdd(a.tie_count, (unsigned)__popcll(tm));
+                tbase = __shfl(tbase, leader);
+                const unsigned slot = tbase + __builtin_amdgcn_mbcnt_hi(
+                                                  (unsigned)(tm >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)tm, 0u));
                 if (tie && slot < a.tie_cap) {
                     a.tie_list[slot] = (unsigned)(((long long)blockIdx.x + qs * G) * n + i);
                     deferred = true;
                 }
-                tm = __ballot(tie && !deferred);
-            }
-            while (tm) {
-                // The reference order: std::sort of the full joint row, eps
-                // over positions 1..k (mutual_information_cpu.cpp:48-77).
-                const int t = __builtin_amdgcn_readfirstlane(__ffsll((long long)tm) - 1);
-                float tqa[D], tqb[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    tqa[d] = __shfl(qa[d], t);
-                    tqb[d] = __shfl(qb[d], t);
-                }
-                const int trec = __shfl(rec, t);
-                for (int j = lane; j < n; j += kWave) {
-                    float va[D], vb[D];
-                    fetch<D>(lds + trec, j, va, vb);
-                    KeyIdx r;
-                    r.key = fmaxf(marg<D>(tqa, va), marg<D>(tqb, vb));
-                    r.idx = j;
-                    scratch[j] = r;
-                }
-                ksg::wave_sync_global();
-                int rjx = 0, rjy = 0;
-                if (lane == 0) {
-                    sort_prefix(scratch, n, K + 1);
-                    float ex = -1.0f, ey = -1.0f;
-                    for (int m = 1; m <= K; ++m) {
-                        const int j = scratch[m].idx;
-                        float va[D], vb[D];
-                        fetch<D>(lds + trec, j, va, vb);
-                        const float da = marg<D>(tqa, va), db = marg<D>(tqb, vb);
-                        rjx = da > ex ? j : rjx;
-                        ex = fmaxf(ex, da);
-                        rjy = db > ey ? j : rjy;
-                        ey = fmaxf(ey, db);
-                    }
-                }
-                rjx = __shfl(rjx, 0);
-                rjy = __shfl(rjy, 0);
-                jx = lane == t ? rjx : jx;
-                jy = lane == t ? rjy : jy;
-                ksg::wave_sync_global();
-                tm &= tm - 1;
             }
             if (res) {
                 if (!deferred) {
-                    pend_x = a.rank[offA + jx];
-                    pend_y = a.rank[offB + jy];
+                    // (0 - 1) | (0 - 1) << 16 is kTieMark.
+                    pend_x = tie ? 0u : a.rank[offA + jx];
+                    pend_y = tie ? 0u : a.rank[offB + jy];
                     pend_at = ((long long)blockIdx.x + qs * G) * n + i;
                     pend = true;
                 }
@@ -941,10 +907,10 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 state = 1;
             }
         }
-        if (__ballot(state == 1) == 0) {
-            pass = (pass / a.checkpoint + 1) * a.checkpoint - 1;  // on to the checkpoint
-            continue;
-        }
+        // No lane walks: the body below changes nothing (one path into the
+        // next pass keeps the loop's registers in place), and the next
+        // pass is a checkpoint.
+        if (__ballot(state == 1) == 0) pass = (pass / a.checkpoint + 1) * a.checkpoint - 1;
         // A body: the next 8 entries of one list. The lane takes the list
         // whose walk is expected to end sooner: after the first block of each,
         // a list's remaining entries below h[k] are estimated as
@@ -974,7 +940,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 useA = (float)posA * (pa - 1.0f) <= (float)posB * (pb - 1.0f);
             }
         }
-        const uint4 blk = useA ? curA : curB;
+        const uint4 blk = sel4(useA, curA, curB);
         const long long off = useA ? offA : offB;
         const long long lastO = useA ? lastB : lastA;  // fixed during the body
         const int posO = useA ? posB : posA;
@@ -987,45 +953,41 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         bool on = state == 1;
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
-            if (on) {
-                const int j = block_entry(blk, st);
-                ++posL;
-                float u[D], v[D];
-                fetch<D>(lds + rec, j, u, v);
-                const float da = marg<D>(qa, u), db = marg<D>(qb, v);
-                const float keyL = useA ? da : db, keyO = useA ? db : da;
-                // Already evaluated if the other list has walked it.
-                const bool seen = lex(keyO, j) <= lastO;
-                lastL = lex(keyL, j);
-                const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
-                insert_packed<K>(h, (seen || j == i) ? 0xffffffffu : c);
-                // Done when the stopping rule holds (in codes: a larger code
-                // means a larger distance; a missing last key reads as NaN,
-                // which fmaxf ignores) or both lists are exhausted; this
-                // body ends too when this list is.
-                const unsigned stop = key_code(fmaxf(keyL, lex_key(lastO)), base, shift) << idb;
-                const bool fin = (stop > h[K]) | ((posL >= n) & (posO >= n));
-                on = !fin & (posL < n);
-                state = fin ? 0 : state;
-                res = res | fin;
-            }
+            // Every lane runs every step (no branches: the loop's registers
+            // stay in place); a lane that is not walking inserts nothing and
+            // keeps its positions. Its block entries are frames of an
+            // earlier walk or 0, so every read stays within its records.
+            const int j = block_entry(blk, st);
+            float u[D], v[D];
+            fetch<D>(lds + rec, j, u, v);
+            const float da = marg<D>(qa, u), db = marg<D>(qb, v);
+            const float keyL = useA ? da : db, keyO = useA ? db : da;
+            // Already evaluated if the other list has walked it.
+            const bool seen = lex(keyO, j) <= lastO;
+            const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
+            insert_packed<K>(h, (!on || seen || j == i) ? 0xffffffffu : c);
+            posL += on ? 1 : 0;
+            lastL = on ? lex(keyL, j) : lastL;
+            // Done when the stopping rule holds (in codes: a larger code
+            // means a larger distance; a missing last key reads as NaN,
+            // which fmaxf ignores) or both lists are exhausted; this body
+            // ends too when this list is.
+            const unsigned stop = key_code(fmaxf(keyL, lex_key(lastO)), base, shift) << idb;
+            const bool fin = on & ((stop > h[K]) | ((posL >= n) & (posO >= n)));
+            res = res | fin;
+            on = on & !fin & (posL < n);
         }
-        if (useA) {
-            lastA = lastL;
-            posA = posL;
-        } else {
-            lastB = lastL;
-            posB = posL;
-        }
-        if (state == 1) {
-            if (useA) {
-                curA = nxtA;
-                nxtA = tmp;
-            } else {
-                curB = nxtB;
-                nxtB = tmp;
-            }
-        }
+        state = (state == 1 && res) ? 0 : state;
+        lastA = useA ? lastL : lastA;
+        posA = useA ? posL : posA;
+        lastB = useA ? lastB : lastL;
+        posB = useA ? posB : posL;
+        // Lanes that are not walking rotate too: a walk's first blocks are
+        // loaded when it starts.
+        curA = sel4(useA, nxtA, curA);
+        nxtA = sel4(useA, tmp, nxtA);
+        curB = sel4(useA, curB, nxtB);
+        nxtB = sel4(useA, nxtB, tmp);
     }
     if (pend) a.cnt[pend_at] = (pend_x - 1u) | ((pend_y - 1u) << 16);
     // Per-wave totals.
@@ -1038,147 +1000,178 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     }
 }
 
-// Resolve the queries the walk deferred (equal k-th and (k+1)-th heap
-// codes): one wave per entry rebuilds the query's joint distance row. When
-// the k-th and (k+1)-th smallest distances over the other frames differ
-// (the codes only collided), the k nearest frames are those with distance
-// <= the k-th, as in the walk. Otherwise the row's positions 0..k are
-// selected in libstdc++ std::sort order (mutual_information_cpu.cpp:48-77)
-// and eps taken over positions 1..k. The counts come from the rank tables.
+// Counts of one deferred query (local pair pl, frame i; equal k-th and
+// (k+1)-th heap codes), by one wave with an n-record LDS row: the wave
+// rebuilds the query's joint distance row. When the k-th and (k+1)-th
+// smallest distances over the other frames differ (the codes only
+// collided), the k nearest frames are those with distance <= the k-th, as
+// in the walk. Otherwise the row's positions 0..k are selected in libstdc++
+// std::sort order (mutual_information_cpu.cpp:48-77) and eps taken over
+// positions 1..k. The counts come from the rank tables.
 template <int D, int K>
-__global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
-    // Each wave's row of n records lives in LDS: the selection's partition
-    // loops are chains of dependent reads.
-    extern __shared__ __attribute__((aligned(16))) KeyIdx tie_rows[];
-    const int lane = threadIdx.x & (kWave - 1);
-    const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
+__device__ void resolve_tie(const WalkArgs &a, long long pl, int i, KeyIdx *row, int lane) {
     const int n = a.n;
-    KeyIdx *scratch = tie_rows + (threadIdx.x / kWave) * n;
-    const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
-    for (long long e = wave; e < count; e += nwaves) {
-        const unsigned v = a.tie_list[e];
-        const long long pl = v / (unsigned)n;
-        const int i = (int)(v - pl * n);
-        const long long p = a.p0 + pl;
-        const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
-        const float *xa = a.X + (long long)na * a.ld;
-        const float *xb = a.X + (long long)nb * a.ld;
-        float qa[D], qb[D];
+    const long long p = a.p0 + pl;
+    const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
+    const float *xa = a.X + (long long)na * a.ld;
+    const float *xb = a.X + (long long)nb * a.ld;
+    float qa[D], qb[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        qa[d] = xa[d * a.dstride + i];
+        qb[d] = xb[d * a.dstride + i];
+    }
+    // The row (self included, as the reference sorts it) and each lane's
+    // k+1 smallest distances over the other frames.
+    float h[K + 1];
+#pragma unroll
+    for (int m = 0; m <= K; ++m) h[m] = __builtin_inff();
+    for (int j = lane; j < n; j += kWave) {
+        float va[D], vb[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            qa[d] = xa[d * a.dstride + i];
-            qb[d] = xb[d * a.dstride + i];
+            va[d] = xa[d * a.dstride + j];
+            vb[d] = xb[d * a.dstride + j];
         }
-        // The row (self included, as the reference sorts it) and each
-        // lane's k+1 smallest distances over the other frames.
-        float h[K + 1];
+        KeyIdx r;
+        r.key = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
+        r.idx = j;
+        row[j] = r;
+        const float c = j == i ? __builtin_inff() : r.key;
 #pragma unroll
-        for (int m = 0; m <= K; ++m) h[m] = __builtin_inff();
+        for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+        h[0] = fminf(h[0], c);
+    }
+    // Merge across the wave: afterwards every lane holds the row's k+1
+    // smallest.
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        float o[K + 1];
+#pragma unroll
+        for (int m = 0; m <= K; ++m) o[m] = __shfl_xor(h[m], off);
+#pragma unroll
+        for (int t = 0; t <= K; ++t) {
+#pragma unroll
+            for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], o[t], h[m]);
+            h[0] = fminf(h[0], o[t]);
+        }
+    }
+    ksg::wave_sync();
+    int jx = 0, jy = 0;
+    if (h[K - 1] < h[K]) {
+        float ex = -1.0f, ey = -1.0f;
         for (int j = lane; j < n; j += kWave) {
+            if (j == i || !(row[j].key <= h[K - 1])) continue;
             float va[D], vb[D];
 #pragma unroll
             for (int d = 0; d < D; ++d) {
                 va[d] = xa[d * a.dstride + j];
                 vb[d] = xb[d * a.dstride + j];
             }
-            KeyIdx r;
-            r.key = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
-            r.idx = j;
-            scratch[j] = r;
-            const float c = j == i ? __builtin_inff() : r.key;
-#pragma unroll
-            for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
-            h[0] = fminf(h[0], c);
+            const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+            jx = da > ex ? j : jx;
+            ex = fmaxf(ex, da);
+            jy = db > ey ? j : jy;
+            ey = fmaxf(ey, db);
         }
-        // Merge across the wave: afterwards every lane holds the row's k+1
-        // smallest.
+        // Any frame of the largest marginal distance gives the same rank.
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
-            float o[K + 1];
-#pragma unroll
-            for (int m = 0; m <= K; ++m) o[m] = __shfl_xor(h[m], off);
-#pragma unroll
-            for (int t = 0; t <= K; ++t) {
-#pragma unroll
-                for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], o[t], h[m]);
-                h[0] = fminf(h[0], o[t]);
-            }
+            const float oex = __shfl_xor(ex, off), oey = __shfl_xor(ey, off);
+            const int ojx = __shfl_xor(jx, off), ojy = __shfl_xor(jy, off);
+            jx = oex > ex ? ojx : jx;
+            ex = fmaxf(ex, oex);
+            jy = oey > ey ? ojy : jy;
+            ey = fmaxf(ey, oey);
         }
-        ksg::wave_sync();
-        int jx = 0, jy = 0;
-        if (h[K - 1] < h[K]) {
-            float ex = -1.0f, ey = -1.0f;
-            for (int j = lane; j < n; j += kWave) {
-                if (j == i || !(scratch[j].key <= h[K - 1])) continue;
-                float va[D], vb[D];
+    } else if (lane == 0) {
+        sort_prefix(row, n, K + 1);
+        float ex = -1.0f, ey = -1.0f;
+        for (int m = 1; m <= K; ++m) {
+            const int j = row[m].idx;
+            float va[D], vb[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    va[d] = xa[d * a.dstride + j];
-                    vb[d] = xb[d * a.dstride + j];
-                }
-                const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
-                jx = da > ex ? j : jx;
-                ex = fmaxf(ex, da);
-                jy = db > ey ? j : jy;
-                ey = fmaxf(ey, db);
+            for (int d = 0; d < D; ++d) {
+                va[d] = xa[d * a.dstride + j];
+                vb[d] = xb[d * a.dstride + j];
             }
-            // Any frame of the largest marginal distance gives the same rank.
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const float oex = __shfl_xor(ex, off), oey = __shfl_xor(ey, off);
-                const int ojx = __shfl_xor(jx, off), ojy = __shfl_xor(jy, off);
-                jx = oex > ex ? ojx : jx;
-                ex = fmaxf(ex, oex);
-                jy = oey > ey ? ojy : jy;
-                ey = fmaxf(ey, oey);
-            }
-        } else if (lane == 0) {
-            sort_prefix(scratch, n, K + 1);
-            float ex = -1.0f, ey = -1.0f;
-            for (int m = 1; m <= K; ++m) {
-                const int j = scratch[m].idx;
-                float va[D], vb[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    va[d] = xa[d * a.dstride + j];
-                    vb[d] = xb[d * a.dstride + j];
-                }
-                const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
-                jx = da > ex ? j : jx;
-                ex = fmaxf(ex, da);
-                jy = db > ey ? j : jy;
-                ey = fmaxf(ey, db);
-            }
+            const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+            jx = da > ex ? j : jx;
+            ex = fmaxf(ex, da);
+            jy = db > ey ? j : jy;
+            ey = fmaxf(ey, db);
         }
-        if (lane == 0) {
-            const long long offA = ((long long)a.slot_of[na] * n + i) * a.rs;
-            const long long offB = ((long long)a.slot_of[nb] * n + i) * a.rs;
-            const unsigned rx = a.rank[offA + jx], ry = a.rank[offB + jy];
-            a.cnt[pl * n + i] = (rx - 1u) | ((ry - 1u) << 16);
+    }
+    if (lane == 0) {
+        const long long offA = ((long long)a.slot_of[na] * n + i) * a.rs;
+        const long long offB = ((long long)a.slot_of[nb] * n + i) * a.rs;
+        const unsigned rx = a.rank[offA + jx], ry = a.rank[offB + jy];
+        a.cnt[pl * n + i] = (rx - 1u) | ((ry - 1u) << 16);
+    }
+    ksg::wave_sync();
+}
+
+// The tie list's queries, one wave each. Each wave's row of n records lives
+// in LDS (the selection's partition loops are chains of dependent reads).
+template <int D, int K>
+__global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
+    extern __shared__ __attribute__((aligned(16))) KeyIdx tie_rows[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
+    KeyIdx *row = tie_rows + (threadIdx.x / kWave) * a.n;
+    const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
+    for (long long e = wave; e < count; e += nwaves) {
+        const unsigned v = a.tie_list[e];
+        const long long pl = v / (unsigned)a.n;
+        resolve_tie<D, K>(a, pl, (int)(v - pl * a.n), row, lane);
+    }
+}
+
+// The tied queries the tie list had no room for (kTieMark in cnt): nothing
+// to do unless the list overflowed. Each wave scans 64-entry spans of cnt.
+template <int D, int K>
+__global__ __launch_bounds__(256) void tie_overflow_kernel(WalkArgs a) {
+    if (*a.tie_count <= a.tie_cap) return;
+    extern __shared__ __attribute__((aligned(16))) KeyIdx tie_rows[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
+    KeyIdx *row = tie_rows + (threadIdx.x / kWave) * a.n;
+    const long long total = a.P * a.n;
+    for (long long s0 = wave * kWave; s0 < total; s0 += nwaves * kWave) {
+        const long long e = s0 + lane;
+        unsigned long long marked = __ballot(e < total && a.cnt[e] == kTieMark);
+        while (marked) {
+            const int t = __ffsll((long long)marked) - 1;
+            const long long q = s0 + t;
+            const long long pl = q / a.n;
+            resolve_tie<D, K>(a, pl, (int)(q - pl * a.n), row, lane);
+            marked &= marked - 1;
         }
-        ksg::wave_sync();
     }
 }
 
 template <int D, int K>
 struct WalkInstances {
-    static void fill(WalkFn *t, TieFn *u) {
+    static void fill(WalkFn *t, TieFn *u, TieFn *o) {
         t[K] = walk_kernel<D, K>;
         u[K] = tie_kernel<D, K>;
-        WalkInstances<D, K - 1>::fill(t, u);
+        o[K] = tie_overflow_kernel<D, K>;
+        WalkInstances<D, K - 1>::fill(t, u, o);
     }
 };
 template <int D>
 struct WalkInstances<D, 0> {
-    static void fill(WalkFn *, TieFn *) {}
+    static void fill(WalkFn *, TieFn *, TieFn *) {}
 };
 
 template <int D>
 struct WalkTable {
     WalkFn t[ksg::kMaxK + 1] = {};
     TieFn u[ksg::kMaxK + 1] = {};
-    WalkTable() { WalkInstances<D, ksg::kMaxK>::fill(t, u); }
+    TieFn o[ksg::kMaxK + 1] = {};
+    WalkTable() { WalkInstances<D, ksg::kMaxK>::fill(t, u, o); }
 };
 
 template <int D>
@@ -1193,8 +1186,9 @@ WalkFn walk_lookup(int k) {
 }
 
 template <int D>
-TieFn tie_lookup(int k) {
-    return (k >= 1 && k <= ksg::kMaxK) ? walk_table<D>().u[k] : nullptr;
+TieFn tie_lookup(int k, bool overflow) {
+    if (k < 1 || k > ksg::kMaxK) return nullptr;
+    return overflow ? walk_table<D>().o[k] : walk_table<D>().u[k];
 }
 
 template <int D>

@@ -345,6 +345,21 @@ def test_list_path_walk_shapes(method_variant, waves, batch):
     assert ties > 0
 
 
+def test_list_path_tie_overflow(method_variant):
+    """More tied queries than the tie list holds (at least 1024 entries, or
+    1/32 of the queries): the rest are marked in the counts and resolved by
+    the overflow pass, with the same bits."""
+    if method_variant != "lists":
+        pytest.skip("the tie list only exists on the list path")
+    n, d, k = 400, 3, 6
+    _flush("tie list overflow")
+    rng = np.random.default_rng(43)
+    X = np.round(rng.standard_normal((4, d * n))).astype(np.float32)
+    ab = np.array([[0, 1], [1, 2], [2, 3], [3, 0], [0, 2], [1, 3]], np.int32)
+    ties = _check_pairs(X, ab, k, n, d, 2)
+    assert ties > 1024
+
+
 def test_list_path_sparse_node_indices(method_variant):
     """Pairs over a few scattered rows of a tall X: the tables cover only the
     used nodes (slot map), and entries walked are reported."""
