@@ -448,11 +448,14 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
         for (int r0 = 0; r0 < R; r0 += 8) {
             const long long e0 = (long long)lane * R + r0;
             if (e0 < a.rs) {
+                // Padding positions (past n) hold frame n - 1: the walk reads
+                // whole blocks.
+                const unsigned last = (unsigned)n - 1u;
                 uint4 w;
-                w.x = (v[r0 + 0] & 0x7ffu) | ((v[r0 + 1] & 0x7ffu) << 16);
-                w.y = (v[r0 + 2] & 0x7ffu) | ((v[r0 + 3] & 0x7ffu) << 16);
-                w.z = (v[r0 + 4] & 0x7ffu) | ((v[r0 + 5] & 0x7ffu) << 16);
-                w.w = (v[r0 + 6] & 0x7ffu) | ((v[r0 + 7] & 0x7ffu) << 16);
+                w.x = min(v[r0 + 0] & 0x7ffu, last) | (min(v[r0 + 1] & 0x7ffu, last) << 16);
+                w.y = min(v[r0 + 2] & 0x7ffu, last) | (min(v[r0 + 3] & 0x7ffu, last) << 16);
+                w.z = min(v[r0 + 4] & 0x7ffu, last) | (min(v[r0 + 5] & 0x7ffu, last) << 16);
+                w.w = min(v[r0 + 6] & 0x7ffu, last) | (min(v[r0 + 7] & 0x7ffu, last) << 16);
                 *reinterpret_cast<uint4 *>(orow + e0) = w;
             }
         }
@@ -547,11 +550,18 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         for (int r0 = 0; r0 < 32; r0 += 8) {
             const long long e0 = (long long)ebase + lane * 32 + r0;
             if (row && e0 < a.rs) {
+                // Padding positions (past n) hold frame n - 1: the walk reads
+                // whole blocks.
+                const unsigned last = (unsigned)n - 1u;
                 uint4 w;
-                w.x = (unsigned)(v[r0 + 0] & 0xffff) | ((unsigned)(v[r0 + 1] & 0xffff) << 16);
-                w.y = (unsigned)(v[r0 + 2] & 0xffff) | ((unsigned)(v[r0 + 3] & 0xffff) << 16);
-                w.z = (unsigned)(v[r0 + 4] & 0xffff) | ((unsigned)(v[r0 + 5] & 0xffff) << 16);
-                w.w = (unsigned)(v[r0 + 6] & 0xffff) | ((unsigned)(v[r0 + 7] & 0xffff) << 16);
+                w.x = min((unsigned)(v[r0 + 0] & 0xffff), last) |
+                      (min((unsigned)(v[r0 + 1] & 0xffff), last) << 16);
+                w.y = min((unsigned)(v[r0 + 2] & 0xffff), last) |
+                      (min((unsigned)(v[r0 + 3] & 0xffff), last) << 16);
+                w.z = min((unsigned)(v[r0 + 4] & 0xffff), last) |
+                      (min((unsigned)(v[r0 + 5] & 0xffff), last) << 16);
+                w.w = min((unsigned)(v[r0 + 6] & 0xffff), last) |
+                      (min((unsigned)(v[r0 + 7] & 0xffff), last) << 16);
                 *reinterpret_cast<uint4 *>(orow + e0) = w;
             }
         }
@@ -943,41 +953,50 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         const uint4 blk = sel4(useA, curA, curB);
         const long long off = useA ? offA : offB;
         const long long lastO = useA ? lastB : lastA;  // fixed during the body
-        const int posO = useA ? posB : posA;
         long long lastL = useA ? lastA : lastB;
         int posL = useA ? posA : posB;
         // The chosen list's block two ahead is requested now and used one
         // body later.
         uint4 tmp = {0, 0, 0, 0};  // unused when the list ends before that block
         if (state == 1 && posL + 16 < rs) tmp = load_block(a.ord + off + posL + 16);
-        bool on = state == 1;
+        const bool walks = state == 1;
+        const int rem = n - posL;  // entries left in the chosen list
+        float key7 = 0.0f;         // the chosen list's key of the body's last entry
+        int j7 = 0;
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
             // Every lane runs every step (no branches: the loop's registers
-            // stay in place); a lane that is not walking inserts nothing and
-            // keeps its positions. Its block entries are frames of an
-            // earlier walk or 0, so every read stays within its records.
+            // stay in place); a lane that is not walking, or past its list's
+            // end, inserts nothing. Block entries are frames (the tables'
+            // padding holds frame n - 1; blocks of no walk are 0), so every
+            // read stays within the lane's records.
             const int j = block_entry(blk, st);
             float u[D], v[D];
             fetch<D>(lds + rec, j, u, v);
             const float da = marg<D>(qa, u), db = marg<D>(qb, v);
-            const float keyL = useA ? da : db, keyO = useA ? db : da;
             // Already evaluated if the other list has walked it.
-            const bool seen = lex(keyO, j) <= lastO;
+            const bool seen = lex(useA ? db : da, j) <= lastO;
             const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
-            insert_packed<K>(h, (!on || seen || j == i) ? 0xffffffffu : c);
-            posL += on ? 1 : 0;
-            lastL = on ? lex(keyL, j) : lastL;
-            // Done when the stopping rule holds (in codes: a larger code
-            // means a larger distance; a missing last key reads as NaN,
-            // which fmaxf ignores) or both lists are exhausted; this body
-            // ends too when this list is.
-            const unsigned stop = key_code(fmaxf(keyL, lex_key(lastO)), base, shift) << idb;
-            const bool fin = on & ((stop > h[K]) | ((posL >= n) & (posO >= n)));
-            res = res | fin;
-            on = on & !fin & (posL < n);
+            const bool take = walks & (st < rem) & !seen & (j != i);
+            insert_packed<K>(h, take ? c : 0xffffffffu);
+            if (st == 7) {
+                key7 = useA ? da : db;
+                j7 = j;
+            }
         }
-        state = (state == 1 && res) ? 0 : state;
+        // The stopping rule, once per body: it is monotone (last keys only
+        // grow, h[K] only falls), so a walk that meets it inside the body
+        // meets it here too, and the entries after that point change
+        // nothing (their codes exceed h[K]'s, or they were seen). A list
+        // walked to its end has evaluated every frame. In codes: a larger
+        // code means a larger distance; a missing last key reads as NaN,
+        // which fmaxf ignores.
+        posL += walks ? min(rem, 8) : 0;
+        lastL = walks ? lex(key7, j7) : lastL;
+        const unsigned stop = key_code(fmaxf(key7, lex_key(lastO)), base, shift) << idb;
+        const bool fin = walks & ((rem <= 8) | (stop > h[K]));
+        res = res | fin;
+        state = fin ? 0 : state;
         lastA = useA ? lastL : lastA;
         posA = useA ? posL : posA;
         lastB = useA ? lastB : lastL;

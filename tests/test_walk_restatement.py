@@ -112,10 +112,11 @@ def _walk_counts(xa, xb, n, d, k, bug=None, shift=None):
                 if not seen and j != i:
                     c = (code(max(DA[j], DB[j])) << idb) | int(j)
                     h = sorted(h + [c])[:k + 1]
-                keys = [v[0] for v in last.values() if v[1] >= 0]
-                if (code(max(keys)) << idb) > h[k] or (pos["a"] >= n and pos["b"] >= n):
-                    done = True
-                    break
+            # the stopping rule once per block, as the kernel; a list walked
+            # to its end has evaluated every frame
+            keys = [v[0] for v in last.values() if v[1] >= 0]
+            if pos[L] >= n or (code(max(keys)) << idb) > h[k]:
+                done = True
         if (h[k - 1] ^ h[k]) >> idb == 0 and bug != "collisions_as_distinct":
             # Tie resolution: exact k-th / (k+1)-th distances over the other
             # frames; equal ones take the std::sort order.
