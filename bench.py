@@ -12,7 +12,7 @@ Prints one JSON line with the driver's fields plus:
   roofline     — the dominant kernel's algorithmic FP32 VALU rate against
                  MI355X's non-packed VALU peak (256 CUs x 64 lanes x 2.4 GHz =
                  39.32 T lane-ops/s): on the sorted-list path (the default)
-                 the walk kernel, 4d+4(k+1) lane-ops per list entry walked
+                 the walk kernel, 4d+k+5 lane-ops per list entry walked
                  over its hipEvent-timed duration; on the all-frames path
                  (8d+2)*n*(n-1) lane-ops per pair; traffic from the committed
                  rocprofv3 counter summary when present;
@@ -188,16 +188,19 @@ def main():
     if method == 2:
         # Sorted-list path: the walk kernel dominates. Its algorithmic work
         # per list entry: both marginal distances (2d subtractions, 2(d-1)
-        # maxima), the joint max, the seen test and the (k+1)-slot heap
-        # insertion (k+1 med3 plus 3 selects/compares per slot): 4d+4(k+1).
-        per_entry = 4 * d + 4 * (k + 1)
+        # maxima; absolute values are operand modifiers), the joint max, the
+        # seen test, the distance code (saturating subtract, shift,
+        # shift-or), the select of the entry or the no-op value, and the
+        # (k+1)-slot insertion of 32-bit (code, frame) entries (one med3 per
+        # slot): 4d+k+5.
+        per_entry = 4 * d + k + 5
         walk_ms = phases["walk"]
         achieved = per_entry * list_entries / (walk_ms * 1e-3)
         roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK / 1e12,
                     "unit": "T lane-ops/s", "frac": achieved / VALU_PEAK,
                     "traffic": traffic_from_profile("ksg_walk_c1_traffic.json"),
                     "kernel": f"walk_kernel<{d}, {k}>", "kernel_ms": walk_ms,
-                    "work": f"4d+4(k+1) = {per_entry} FP32 VALU lane-ops per list entry walked, "
+                    "work": f"4d+k+5 = {per_entry} VALU lane-ops per list entry walked, "
                             f"{list_entries} entries"}
     else:
         roofline = {"bound": "valu", "achieved": equivalent / 1e12, "peak": VALU_PEAK / 1e12,
