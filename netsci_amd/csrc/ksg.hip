@@ -667,15 +667,16 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
                                wa);
             NSX_HIP(hipGetLastError(), "walk launch");
             if (e1) (void)hipEventRecord(e1, stream);
-            // Up to 4 waves per workgroup, each with an n-record LDS row.
-            const int tie_wpb = (int)std::max<size_t>(
-                1, std::min<size_t>(4, lds_cap / ((size_t)n * sizeof(KeyIdx))));
+            // Up to 4 waves per workgroup, each with its LDS row and lists.
+            const size_t tie_wave_bytes = ksgw::tie_wave_lds_bytes(n);
+            const int tie_wpb =
+                (int)std::max<size_t>(1, std::min<size_t>(4, lds_cap / tie_wave_bytes));
             hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / tie_wpb)),
-                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * n * sizeof(KeyIdx),
+                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
                                stream, wa);
             NSX_HIP(hipGetLastError(), "tie launch");
             hipLaunchKernelGGL(tieo, dim3((unsigned)(tie_waves / tie_wpb)),
-                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * n * sizeof(KeyIdx),
+                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
                                stream, wa);
             NSX_HIP(hipGetLastError(), "tie overflow launch");
             if (e2) (void)hipEventRecord(e2, stream);

@@ -136,6 +136,13 @@ inline size_t walk_lds_bytes(int d, int n, int nbuf) {
     return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 5 * nbuf) * sizeof(int);
 }
 
+// LDS bytes per wave of the tie kernels: the row (n records), the two
+// stopper position lists of a partition (n 16-bit entries each) and the
+// sub-range stack (64 x 3 ints).
+NSX_HD inline size_t tie_wave_lds_bytes(int n) {
+    return (size_t)n * sizeof(KeyIdx) + (size_t)4 * n + 64 * 3 * sizeof(int);
+}
+
 // Heap entries of the walk: (distance code << id_bits) | frame in 32 bits,
 // id_bits the frame bits for n frames. The code covers the 2^27 float-bit
 // values (16 binades) below the pair's coordinate span at 2^code_shift per
@@ -1019,6 +1026,122 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     }
 }
 
+// libstdc++'s unguarded partition of a[f1, l) around the pivot record a[f1 - 1]
+// (partition_unguarded in sort_select.h), by one wave. The sequential scans
+// stop at the "left stoppers" (positions p >= f1 with !(a[p] < pivot)) from
+// the left and at the "right stoppers" (p >= f1 - 1 with !(pivot < a[p]); the
+// pivot is one) from the right. Until the scans cross, the t-th swap
+// exchanges the t-th left stopper Ls[t] with the t-th right stopper Rs[t]
+// counted from the right, all in unswapped territory. The loop ends at the
+// first t with Ls[t] >= Rs[t] (or no such stopper) and returns min(Ls[t],
+// Rs[t - 1]), with Rs[0] = l. So: list both kinds of stoppers by prefix
+// counts, find that t, and do the t - 1 swaps at once (their positions are
+// distinct). Returns the cut, wave-uniform.
+__device__ int partition_wave(KeyIdx *a, int f1, int l, uint16_t *lasc, uint16_t *rasc, int lane) {
+    const float pivot = a[f1 - 1].key;
+    int nl = 0, nr = 0;
+    for (int c0 = f1 - 1; c0 < l; c0 += kWave) {
+        const int p = c0 + lane;
+        const bool in = p < l;
+        const float v = in ? a[p].key : 0.0f;
+        const bool isl = in && p >= f1 && !(v < pivot);
+        const bool isr = in && !(pivot < v);
+        const unsigned long long ml = __ballot(isl), mr = __ballot(isr);
+        const unsigned bl = __builtin_amdgcn_mbcnt_hi((unsigned)(ml >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)ml, 0u));
+        const unsigned br = __builtin_amdgcn_mbcnt_hi((unsigned)(mr >> 32),
+                                                      __builtin_amdgcn_mbcnt_lo((unsigned)mr, 0u));
+        if (isl) lasc[nl + bl] = (uint16_t)p;
+        if (isr) rasc[nr + br] = (uint16_t)p;
+        nl += __popcll(ml);
+        nr += __popcll(mr);
+    }
+    ksg::wave_sync();
+    // Ls[t] = lasc[t - 1], Rs[t] = rasc[nr - t].
+    int T = 0;
+    for (int t0 = 1; T == 0; t0 += kWave) {
+        const int t = t0 + lane;
+        const bool stop = t > nl || t > nr || lasc[t - 1] >= rasc[nr - t];
+        const unsigned long long ms = __ballot(stop);
+        if (ms) T = t0 + __ffsll((long long)ms) - 1;
+    }
+    for (int t0 = 1; t0 < T; t0 += kWave) {
+        const int t = t0 + lane;
+        if (t < T) {
+            const int pl = lasc[t - 1], pr = rasc[nr - t];
+            const KeyIdx x = a[pl], y = a[pr];
+            a[pl] = y;
+            a[pr] = x;
+        }
+    }
+    const int ls = T <= nl ? (int)lasc[T - 1] : 0x7fffffff;
+    const int rs_prev = T >= 2 ? (int)rasc[nr - (T - 1)] : l;
+    ksg::wave_sync();
+    return min(ls, rs_prev);
+}
+
+// sort_prefix (sort_select.h) by one wave: the same sub-range steps in the
+// same order, with each partition of more than 128 records done by
+// partition_wave and the small steps (median of three, short partitions,
+// the heap-sort fallback, the final insertion sort) by lane 0. a[0, m) ends
+// as std::sort leaves it.
+__device__ int sort_prefix_wave(KeyIdx *a, int n, int m, uint16_t *lasc, uint16_t *rasc,
+                                int *stk, int lane) {
+    if (n <= 0) return 0;
+    if (m > n) m = n;
+    constexpr int kThreshold = 16, kStack = 64, kWaveMin = 128;
+    int bound = n;
+    if (lane == 0) {
+        stk[0] = 0;
+        stk[1] = n;
+        stk[2] = 2 * floor_log2(n);
+    }
+    ksg::wave_sync();
+    int sp = 1;
+    while (sp > 0) {
+        --sp;
+        int f = __builtin_amdgcn_readfirstlane(stk[3 * sp]);
+        int l = __builtin_amdgcn_readfirstlane(stk[3 * sp + 1]);
+        int depth = __builtin_amdgcn_readfirstlane(stk[3 * sp + 2]);
+        while (l - f > kThreshold) {
+            if (depth == 0) {
+                if (lane == 0) heap_sort_range(a + f, l - f);
+                ksg::wave_sync();
+                break;
+            }
+            --depth;
+            const int mid = f + (l - f) / 2;
+            int cut = 0;
+            if (lane == 0) median_to_first(a, f, f + 1, mid, l - 1);
+            ksg::wave_sync();
+            if (l - f > kWaveMin) {
+                cut = partition_wave(a, f + 1, l, lasc, rasc, lane);
+            } else {
+                if (lane == 0) cut = (int)partition_unguarded(a, f + 1, l, f);
+                cut = __shfl(cut, 0);
+                ksg::wave_sync();
+            }
+            cut = __builtin_amdgcn_readfirstlane(cut);
+            if (cut >= m) {
+                bound = min(bound, cut);
+            } else if (sp < kStack) {
+                // The right part still reaches [0, m): process it too.
+                if (lane == 0) {
+                    stk[3 * sp] = cut;
+                    stk[3 * sp + 1] = l;
+                    stk[3 * sp + 2] = depth;
+                }
+                ksg::wave_sync();
+                ++sp;
+            }
+            l = cut;
+        }
+    }
+    if (lane == 0) insertion_sort(a, bound);
+    ksg::wave_sync();
+    return bound;
+}
+
 // Counts of one deferred query (local pair pl, frame i; equal k-th and
 // (k+1)-th heap codes), by one wave with an n-record LDS row: the wave
 // rebuilds the query's joint distance row. When the k-th and (k+1)-th
@@ -1028,7 +1151,11 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
 // std::sort order (mutual_information_cpu.cpp:48-77) and eps taken over
 // positions 1..k. The counts come from the rank tables.
 template <int D, int K>
-__device__ void resolve_tie(const WalkArgs &a, long long pl, int i, KeyIdx *row, int lane) {
+__device__ void resolve_tie(const WalkArgs &a, long long pl, int i, char *wave_lds, int lane) {
+    KeyIdx *row = reinterpret_cast<KeyIdx *>(wave_lds);
+    uint16_t *lasc = reinterpret_cast<uint16_t *>(wave_lds + (size_t)a.n * sizeof(KeyIdx));
+    uint16_t *rasc = lasc + a.n;
+    int *stk = reinterpret_cast<int *>(rasc + a.n);
     const int n = a.n;
     const long long p = a.p0 + pl;
     const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
@@ -1103,8 +1230,8 @@ __device__ void resolve_tie(const WalkArgs &a, long long pl, int i, KeyIdx *row,
             jy = oey > ey ? ojy : jy;
             ey = fmaxf(ey, oey);
         }
-    } else if (lane == 0) {
-        sort_prefix(row, n, K + 1);
+    } else {
+        sort_prefix_wave(row, n, K + 1, lasc, rasc, stk, lane);
         float ex = -1.0f, ey = -1.0f;
         for (int m = 1; m <= K; ++m) {
             const int j = row[m].idx;
@@ -1130,20 +1257,20 @@ __device__ void resolve_tie(const WalkArgs &a, long long pl, int i, KeyIdx *row,
     ksg::wave_sync();
 }
 
-// The tie list's queries, one wave each. Each wave's row of n records lives
-// in LDS (the selection's partition loops are chains of dependent reads).
+// The tie list's queries, one wave each, with tie_wave_lds_bytes(n) of LDS
+// per wave (the row of n records and the partition's lists).
 template <int D, int K>
 __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
-    extern __shared__ __attribute__((aligned(16))) KeyIdx tie_rows[];
+    extern __shared__ __attribute__((aligned(16))) char tie_lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
-    KeyIdx *row = tie_rows + (threadIdx.x / kWave) * a.n;
+    char *wl = tie_lds + (threadIdx.x / kWave) * tie_wave_lds_bytes(a.n);
     const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
     for (long long e = wave; e < count; e += nwaves) {
         const unsigned v = a.tie_list[e];
         const long long pl = v / (unsigned)a.n;
-        resolve_tie<D, K>(a, pl, (int)(v - pl * a.n), row, lane);
+        resolve_tie<D, K>(a, pl, (int)(v - pl * a.n), wl, lane);
     }
 }
 
@@ -1152,11 +1279,11 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
 template <int D, int K>
 __global__ __launch_bounds__(256) void tie_overflow_kernel(WalkArgs a) {
     if (*a.tie_count <= a.tie_cap) return;
-    extern __shared__ __attribute__((aligned(16))) KeyIdx tie_rows[];
+    extern __shared__ __attribute__((aligned(16))) char tie_lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
-    KeyIdx *row = tie_rows + (threadIdx.x / kWave) * a.n;
+    char *wl = tie_lds + (threadIdx.x / kWave) * tie_wave_lds_bytes(a.n);
     const long long total = a.P * a.n;
     for (long long s0 = wave * kWave; s0 < total; s0 += nwaves * kWave) {
         const long long e = s0 + lane;
@@ -1165,7 +1292,7 @@ __global__ __launch_bounds__(256) void tie_overflow_kernel(WalkArgs a) {
             const int t = __ffsll((long long)marked) - 1;
             const long long q = s0 + t;
             const long long pl = q / a.n;
-            resolve_tie<D, K>(a, pl, (int)(q - pl * a.n), row, lane);
+            resolve_tie<D, K>(a, pl, (int)(q - pl * a.n), wl, lane);
             marked &= marked - 1;
         }
     }
