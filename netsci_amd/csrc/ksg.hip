@@ -432,19 +432,31 @@ __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a
     for (int f0 = 0; f0 < a.n; f0 += 64) {
         const int f = f0 + lane;
         const int cols = min(64, a.n - f0);
-        for (int r = 0; r < rows; ++r) {
-            float term = 0.0f;
-            if (f < a.n) {
-                const long long at = (p0 + r) * a.n + f;
-                const unsigned v = a.cnt[at];
-                const int nx = (int)(v & 0xffffu), ny = (int)(v >> 16);
-                term = a.psi[nx] + a.psi[ny];
-                if (a.counts_out) {
-                    a.counts_out[at * 2 + 0] = nx;
-                    a.counts_out[at * 2 + 1] = ny;
+        // 16 rows' counts in flight at a time, then their psi terms.
+        for (int r0 = 0; r0 < rows; r0 += 16) {
+            unsigned v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int r = r0 + u;
+                v[u] = (f < a.n && r < rows) ? a.cnt[(p0 + r) * a.n + f] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int r = r0 + u;
+                if (r < rows) {
+                    float term = 0.0f;
+                    if (f < a.n) {
+                        const int nx = (int)(v[u] & 0xffffu), ny = (int)(v[u] >> 16);
+                        term = a.psi[nx] + a.psi[ny];
+                        if (a.counts_out) {
+                            const long long at = (p0 + r) * a.n + f;
+                            a.counts_out[at * 2 + 0] = nx;
+                            a.counts_out[at * 2 + 1] = ny;
+                        }
+                    }
+                    tile[w][r][lane] = term;
                 }
             }
-            tile[w][r][lane] = term;
         }
         ksg::wave_sync();
         if (lane < rows)

@@ -67,9 +67,10 @@ constexpr int kMaxFrames = 8192;  // 32 sort keys per lane, up to 4 waves per ro
 constexpr int kBuildWaves = 4;
 constexpr int kMaxWalkWaves = 16;
 
-// Waves per walk workgroup a k allows: the heap of k+1 (distance, frame)
-// registers fits 128 VGPRs (16 waves, 4 per SIMD) up to k = 6.
-constexpr int walk_max_waves(int k) { return k <= 6 ? 16 : 8; }
+// Waves per walk workgroup a k allows: with k+1 packed 32-bit heap
+// entries the walk fits 128 VGPRs (16 waves, 4 per SIMD) for every k up to
+// ksg::kMaxK (97 at k = 16).
+constexpr int walk_max_waves(int) { return 16; }
 
 struct BuildArgs {
     const float *X;
@@ -455,14 +456,17 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArg
         for (int r0 = 0; r0 < R; r0 += 8) {
             const long long e0 = (long long)lane * R + r0;
             if (e0 < a.rs) {
-                // Padding positions (past n) hold frame n - 1: the walk reads
-                // whole blocks.
-                const unsigned last = (unsigned)n - 1u;
+                // Padding positions (past n) hold the row's own frame i: the
+                // walk reads whole blocks and never inserts frame i.
+                unsigned e8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    e8[u] = e0 + u < n ? (v[r0 + u] & 0x7ffu) : (unsigned)i;
                 uint4 w;
-                w.x = min(v[r0 + 0] & 0x7ffu, last) | (min(v[r0 + 1] & 0x7ffu, last) << 16);
-                w.y = min(v[r0 + 2] & 0x7ffu, last) | (min(v[r0 + 3] & 0x7ffu, last) << 16);
-                w.z = min(v[r0 + 4] & 0x7ffu, last) | (min(v[r0 + 5] & 0x7ffu, last) << 16);
-                w.w = min(v[r0 + 6] & 0x7ffu, last) | (min(v[r0 + 7] & 0x7ffu, last) << 16);
+                w.x = e8[0] | (e8[1] << 16);
+                w.y = e8[2] | (e8[3] << 16);
+                w.z = e8[4] | (e8[5] << 16);
+                w.w = e8[6] | (e8[7] << 16);
                 *reinterpret_cast<uint4 *>(orow + e0) = w;
             }
         }
@@ -557,18 +561,17 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         for (int r0 = 0; r0 < 32; r0 += 8) {
             const long long e0 = (long long)ebase + lane * 32 + r0;
             if (row && e0 < a.rs) {
-                // Padding positions (past n) hold frame n - 1: the walk reads
-                // whole blocks.
-                const unsigned last = (unsigned)n - 1u;
+                // Padding positions (past n) hold the row's own frame i: the
+                // walk reads whole blocks and never inserts frame i.
+                unsigned e8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    e8[u] = e0 + u < n ? (unsigned)(v[r0 + u] & 0xffff) : (unsigned)i;
                 uint4 w;
-                w.x = min((unsigned)(v[r0 + 0] & 0xffff), last) |
-                      (min((unsigned)(v[r0 + 1] & 0xffff), last) << 16);
-                w.y = min((unsigned)(v[r0 + 2] & 0xffff), last) |
-                      (min((unsigned)(v[r0 + 3] & 0xffff), last) << 16);
-                w.z = min((unsigned)(v[r0 + 4] & 0xffff), last) |
-                      (min((unsigned)(v[r0 + 5] & 0xffff), last) << 16);
-                w.w = min((unsigned)(v[r0 + 6] & 0xffff), last) |
-                      (min((unsigned)(v[r0 + 7] & 0xffff), last) << 16);
+                w.x = e8[0] | (e8[1] << 16);
+                w.y = e8[2] | (e8[3] << 16);
+                w.z = e8[4] | (e8[5] << 16);
+                w.w = e8[6] | (e8[7] << 16);
                 *reinterpret_cast<uint4 *>(orow + e0) = w;
             }
         }
@@ -973,10 +976,11 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
             // Every lane runs every step (no branches: the loop's registers
-            // stay in place); a lane that is not walking, or past its list's
-            // end, inserts nothing. Block entries are frames (the tables'
-            // padding holds frame n - 1; blocks of no walk are 0), so every
-            // read stays within the lane's records.
+            // stay in place); a lane that is not walking inserts nothing.
+            // Block entries are frames (the tables' padding past a list's
+            // end holds the query frame i, which is never inserted; blocks
+            // of no walk are 0), so every read stays within the lane's
+            // records.
             const int j = block_entry(blk, st);
             float u[D], v[D];
             fetch<D>(lds + rec, j, u, v);
@@ -984,7 +988,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             // Already evaluated if the other list has walked it.
             const bool seen = lex(useA ? db : da, j) <= lastO;
             const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
-            const bool take = walks & (st < rem) & !seen & (j != i);
+            const bool take = walks & !seen & (j != i);
             insert_packed<K>(h, take ? c : 0xffffffffu);
             if (st == 7) {
                 key7 = useA ? da : db;
