@@ -178,14 +178,17 @@ TieFn tie_d3(int k, bool overflow);
 
 using ksg::marg;
 
-// The value of lane (lane ^ M): DPP quad permutations for M = 1, 2 (VALU),
-// ds_swizzle in bitmask mode for M = 4, 8, 16, ds_bpermute for M = 32.
+// The value of lane (lane ^ M): DPP for M = 1, 2 (quad permutations) and 8
+// (row rotation by 8 within 16 lanes), all VALU; ds_swizzle in bitmask mode
+// for M = 4, 16, ds_bpermute for M = 32.
 template <int M>
 __device__ __forceinline__ unsigned xor_lane(unsigned v) {
     if constexpr (M == 1) {
         return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
     } else if constexpr (M == 2) {
         return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (M == 8) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, false);  // row_ror:8
     } else if constexpr (M < 32) {
         return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
     } else {
@@ -299,6 +302,10 @@ __device__ __forceinline__ unsigned xor_lane_any(unsigned v) {
         return xor_lane<M>(v);
     } else if constexpr (M == 3) {
         return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x1B, 0xF, 0xF, false);
+    } else if constexpr (M == 7) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    } else if constexpr (M == 15) {
+        return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false);  // row_mirror
     } else if constexpr (M < 32) {
         return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
     } else {
@@ -306,16 +313,52 @@ __device__ __forceinline__ unsigned xor_lane_any(unsigned v) {
     }
 }
 
-// Half-cleaner stages STRIDE, STRIDE/2, .., 1 (all ascending).
+// Half-cleaner stages STRIDE, STRIDE/2, .., 1 (all ascending). Partner
+// lanes at distance 16 or 32: v_permlane16/32_swap of two registers puts
+// both members of each pair of either register in one lane (the lower
+// member in the first result), a min and a max order them, and the same
+// swap puts them back. At distance 4: DPP row_shl:4 (lane l + 4) for the
+// lower lanes, row_shr:4 (lane l - 4) for the upper ones. Both stay off the
+// LDS unit, which the other cross-lane exchanges share with the row loads.
 template <int R, int STRIDE>
 __device__ __forceinline__ void asc_stages(unsigned (&v)[R], int lane) {
     if constexpr (STRIDE >= R) {
         constexpr int LX = STRIDE / R;
-        const bool lower = (lane & LX) == 0;
+        if constexpr (LX == 16 || LX == 32) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const unsigned o = xor_lane<LX>(v[r]);
-            v[r] = lower ? min(v[r], o) : max(v[r], o);
+            for (int r = 0; r < R; r += 2) {
+                unsigned lo, hi;
+                if constexpr (LX == 16) {
+                    const auto t = __builtin_amdgcn_permlane16_swap(v[r], v[r + 1], false, false);
+                    lo = min((unsigned)t[0], (unsigned)t[1]);
+                    hi = max((unsigned)t[0], (unsigned)t[1]);
+                    const auto u = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+                    v[r] = u[0];
+                    v[r + 1] = u[1];
+                } else {
+                    const auto t = __builtin_amdgcn_permlane32_swap(v[r], v[r + 1], false, false);
+                    lo = min((unsigned)t[0], (unsigned)t[1]);
+                    hi = max((unsigned)t[0], (unsigned)t[1]);
+                    const auto u = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+                    v[r] = u[0];
+                    v[r + 1] = u[1];
+                }
+            }
+        } else if constexpr (LX == 4) {
+            const bool lower = (lane & 4) == 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const unsigned up = (unsigned)__builtin_amdgcn_mov_dpp((int)v[r], 0x104, 0xF, 0xF, false);
+                const unsigned dn = (unsigned)__builtin_amdgcn_mov_dpp((int)v[r], 0x114, 0xF, 0xF, false);
+                v[r] = lower ? min(v[r], up) : max(v[r], dn);
+            }
+        } else {
+            const bool lower = (lane & LX) == 0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const unsigned o = xor_lane<LX>(v[r]);
+                v[r] = lower ? min(v[r], o) : max(v[r], o);
+            }
         }
     } else {
 #pragma unroll
