@@ -880,7 +880,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         if ((want | ready) &&
             (__popcll(want) + __popcll(ready) >= (unsigned)a.batch || walking == 0)) {
             // (0) counts looked up in the previous round.
-            if (pend) a.cnt[pend_at] = (pend_x - 1u) | ((pend_y - 1u) << 16);
+            if (pend)
+                __builtin_nontemporal_store((pend_x - 1u) | ((pend_y - 1u) << 16), a.cnt + pend_at);
             pend = false;
             // (1) eps of the finished walks.
             const bool tie = res && ((h[K - 1] ^ h[K]) >> idb) == 0u;
@@ -921,8 +922,10 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             if (res) {
                 if (!deferred) {
                     // (0 - 1) | (0 - 1) << 16 is kTieMark.
-                    pend_x = tie ? 0u : a.rank[offA + jx];
-                    pend_y = tie ? 0u : a.rank[offB + jy];
+                    // Non-temporal: each rank line serves one lookup, and
+                    // keeping it out of L2 leaves room for the list rows.
+                    pend_x = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + offA + jx);
+                    pend_y = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + offB + jy);
                     pend_at = ((long long)blockIdx.x + qs * G) * n + i;
                     pend = true;
                 }
@@ -1062,7 +1065,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         curB = sel4(useA, curB, nxtB);
         nxtB = sel4(useA, nxtB, tmp);
     }
-    if (pend) a.cnt[pend_at] = (pend_x - 1u) | ((pend_y - 1u) << 16);
+    if (pend) __builtin_nontemporal_store((pend_x - 1u) | ((pend_y - 1u) << 16), a.cnt + pend_at);
     // Per-wave totals.
     unsigned long long sum = steps;
 #pragma unroll
