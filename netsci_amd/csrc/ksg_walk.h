@@ -890,7 +890,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 float ex = -1.0f, ey = -1.0f;
 #pragma unroll
                 for (int m = 0; m < K; ++m) {
-                    const int j = (int)(h[m] & id_mask);
+                    const int j = (int)(h[m] & id_mask & (kMaxFrames - 1));  // < 8192: 24-bit address math
                     float va[D], vb[D];
                     fetch<D>(lds + rec, j, va, vb);
                     const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
