@@ -360,6 +360,25 @@ def test_list_path_tie_overflow(method_variant):
     assert ties > 1024
 
 
+def test_list_path_extreme_spans(method_variant):
+    """The walk's heap codes sit in a window below each pair's coordinate
+    span: a node with constant coordinates (span 0, every marginal distance
+    0), spans of 1e20 and 1e-20, and pairs mixing them must still give the
+    oracle's counts (code collisions go to the exact tie resolution)."""
+    if method_variant != "lists":
+        pytest.skip("heap codes only exist on the list path")
+    n, d, k = 300, 3, 4
+    _flush("extreme spans")
+    rng = np.random.default_rng(47)
+    X = rng.standard_normal((5, d * n)).astype(np.float32)
+    X[0] = 1.5
+    X[2] *= np.float32(1e20)
+    X[3] *= np.float32(1e-20)
+    X[4] = np.round(X[4] * 2) / 2
+    ab = np.array([[0, 1], [1, 2], [2, 3], [0, 3], [1, 3], [3, 4], [0, 4], [2, 2]], np.int32)
+    _check_pairs(X, ab, k, n, d, 2)
+
+
 def test_list_path_sparse_node_indices(method_variant):
     """Pairs over a few scattered rows of a tall X: the tables cover only the
     used nodes (slot map), and entries walked are reported."""
