@@ -667,7 +667,7 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
             wa.tie_cap = wtie_cap;
             wa.nbuf = nbuf;
             wa.batch = g_walk_batch;
-            wa.checkpoint = 16;
+            wa.checkpoint = 24;
             wa.span = span;
             wa.id_bits = ksgw::heap_id_bits(n);
             wa.code_shift = ksgw::heap_code_shift(n);

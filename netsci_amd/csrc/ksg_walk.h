@@ -868,12 +868,17 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             if (ns > 0) {
                 for (int x = 0; x < ns; ++x) stage_pair<D>(a, lds, meta, to_stage[x], n);
                 __syncthreads();
-                if (threadIdx.x == 0) ctrl[1] += ns;
-                __syncthreads();
+                // Published without a barrier: a wave that reads the old
+                // count starts the new pair's queries one pass later, and
+                // the next checkpoint's barrier orders this write.
+                if (threadIdx.x == 0)
+                    __hip_atomic_fetch_add(ctrl + 1, ns, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             if (ctrl[2] >= npairs) break;
         }
-        const int avail = ctrl[1] * n;  // queries of staged pairs
+        // Queries of staged pairs (the count grows at checkpoints).
+        const int avail =
+            __hip_atomic_load(ctrl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) * n;
         const unsigned long long want = __ballot(state == 0);
         const unsigned long long ready = __ballot(state == 3 && q < avail);
         const unsigned long long walking = __ballot(state == 1);
