@@ -399,8 +399,10 @@ __device__ __forceinline__ void mirror_merges(unsigned (&v)[R], int lane) {
 }
 
 // One workgroup per (node slot, row group); each wave sorts whole rows.
+// Three waves per SIMD: at R = 32 that costs 21 VGPRs of spills and
+// measures 3% faster than two waves without.
 template <int D, int R>
-__global__ __launch_bounds__(kWave *kBuildWaves) void list_build_kernel(BuildArgs a) {
+__global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(BuildArgs a) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const int n = a.n;
     const int slot = blockIdx.x / a.groups;
