@@ -535,7 +535,8 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     // Tables (ord and rank, U x n rows of rs entries each), sized for the
     // longest window and reused by every window.
     const long long rs_max = ((long long)pb.nmax + 7) & ~7LL;
-    const size_t table_bytes = (size_t)U * pb.nmax * rs_max * sizeof(uint16_t);
+    const size_t table_bytes =
+        (size_t)U * (size_t)ksgw::table_rows(pb.nmax) * rs_max * sizeof(uint16_t);
     size_t free_b = 0, total_b = 0;
     NSX_HIP(hipMemGetInfo(&free_b, &total_b), "memory info");
     const double budget = 0.85 * (double)free_b - 512.0 * (1 << 20);

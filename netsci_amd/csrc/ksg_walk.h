@@ -137,6 +137,23 @@ inline size_t walk_lds_bytes(int d, int n, int nbuf) {
     return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 5 * nbuf) * sizeof(int);
 }
 
+// Table layout: the rows of 16 consecutive frames of a node are stored
+// block-interleaved, [slot][frame / 16][position / 8][frame % 16][8], so the
+// lanes that walk consecutive frames of one pair side by side read the same
+// block of their rows from two adjacent 128-byte lines. table_row gives the
+// offset of (slot, frame i)'s position 0, table_at that of position e.
+constexpr int kRowGroup = 16;
+NSX_HD inline long long table_rows(int n) {
+    return ((long long)n + kRowGroup - 1) / kRowGroup * kRowGroup;
+}
+NSX_HD inline long long table_row(int slot, int i, int n, long long rs) {
+    const long long g = (long long)slot * (table_rows(n) / kRowGroup) + i / kRowGroup;
+    return g * rs * kRowGroup + (long long)(i % kRowGroup) * 8;
+}
+NSX_HD inline long long table_at(long long row, long long e) {
+    return row + (e >> 3) * (8 * kRowGroup) + (e & 7);
+}
+
 // LDS bytes per wave of the tie kernels: the row (n records), the two
 // stopper position lists of a partition (n 16-bit entries each) and the
 // sub-range stack (64 x 3 ints).
@@ -496,7 +513,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
         }
 
         // ord row: this lane's R consecutive positions, 16 bytes at a time.
-        uint16_t *orow = a.ord + ((long long)slot * n + i) * a.rs;
+        const long long orow = table_row(slot, i, n, a.rs);
 #pragma unroll
         for (int r0 = 0; r0 < R; r0 += 8) {
             const long long e0 = (long long)lane * R + r0;
@@ -512,7 +529,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
                 w.y = e8[2] | (e8[3] << 16);
                 w.z = e8[4] | (e8[5] << 16);
                 w.w = e8[6] | (e8[7] << 16);
-                *reinterpret_cast<uint4 *>(orow + e0) = w;
+                *reinterpret_cast<uint4 *>(a.ord + table_at(orow, e0)) = w;
             }
         }
 
@@ -544,9 +561,10 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
             if (e < n) st[v[r]] = (uint16_t)min(sfx[r], after);
         }
         ksg::wave_sync();
-        uint16_t *rrow = a.rank + ((long long)slot * n + i) * a.rs;
+        const long long rrow = table_row(slot, i, n, a.rs);
         for (long long c = (long long)lane * 8; c < a.rs; c += kWave * 8)
-            *reinterpret_cast<uint4 *>(rrow + c) = *reinterpret_cast<const uint4 *>(st + c);
+            *reinterpret_cast<uint4 *>(a.rank + table_at(rrow, c)) =
+                *reinterpret_cast<const uint4 *>(st + c);
         ksg::wave_sync();
     }
 }
@@ -601,7 +619,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         wide_merges<W, 4096>(v, lane, ebase, xr);
 
         // ord row.
-        uint16_t *orow = a.ord + ((long long)slot * n + (row ? i : 0)) * a.rs;
+        const long long orow = table_row(slot, row ? i : 0, n, a.rs);
 #pragma unroll
         for (int r0 = 0; r0 < 32; r0 += 8) {
             const long long e0 = (long long)ebase + lane * 32 + r0;
@@ -617,7 +635,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
                 w.y = e8[2] | (e8[3] << 16);
                 w.z = e8[4] | (e8[5] << 16);
                 w.w = e8[6] | (e8[7] << 16);
-                *reinterpret_cast<uint4 *>(orow + e0) = w;
+                *reinterpret_cast<uint4 *>(a.ord + table_at(orow, e0)) = w;
             }
         }
 
@@ -660,9 +678,10 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         }
         __syncthreads();
         if (row) {
-            uint16_t *rrow = a.rank + ((long long)slot * n + i) * a.rs;
+            const long long rrow = table_row(slot, i, n, a.rs);
             for (long long c = ((long long)wl * kWave + lane) * 8; c < a.rs; c += W * kWave * 8)
-                *reinterpret_cast<uint4 *>(rrow + c) = *reinterpret_cast<const uint4 *>(st + c);
+                *reinterpret_cast<uint4 *>(a.rank + table_at(rrow, c)) =
+                    *reinterpret_cast<const uint4 *>(st + c);
         }
         __syncthreads();
     }
@@ -931,8 +950,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                     // (0 - 1) | (0 - 1) << 16 is kTieMark.
                     // Non-temporal: each rank line serves one lookup, and
                     // keeping it out of L2 leaves room for the list rows.
-                    pend_x = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + offA + jx);
-                    pend_y = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + offB + jy);
+                    pend_x = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(offA, jx));
+                    pend_y = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(offB, jy));
                     pend_at = ((long long)blockIdx.x + qs * G) * n + i;
                     pend = true;
                 }
@@ -968,14 +987,14 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 for (int m = 0; m <= K; ++m) h[m] = 0xffffffffu;
                 posA = posB = 0;
                 lastA = lastB = -1;
-                offA = ((long long)meta[3 * b] * n + i) * rs;
-                offB = ((long long)meta[3 * b + 1] * n + i) * rs;
+                offA = table_row(meta[3 * b], i, n, rs);
+                offB = table_row(meta[3 * b + 1], i, n, rs);
                 base = (unsigned)meta[3 * b + 2];
                 curA = load_block(a.ord + offA);
                 curB = load_block(a.ord + offB);
                 if (rs > 8) {
-                    nxtA = load_block(a.ord + offA + 8);
-                    nxtB = load_block(a.ord + offB + 8);
+                    nxtA = load_block(a.ord + table_at(offA, 8));
+                    nxtB = load_block(a.ord + table_at(offB, 8));
                 }
                 state = 1;
             }
@@ -1021,7 +1040,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         // The chosen list's block two ahead is requested now and used one
         // body later.
         uint4 tmp = {0, 0, 0, 0};  // unused when the list ends before that block
-        if (state == 1 && posL + 16 < rs) tmp = load_block(a.ord + off + posL + 16);
+        if (state == 1 && posL + 16 < rs) tmp = load_block(a.ord + table_at(off, posL + 16));
         const bool walks = state == 1;
         const int rem = n - posL;  // entries left in the chosen list
         float key7 = 0.0f;         // the chosen list's key of the body's last entry
@@ -1306,9 +1325,9 @@ __device__ void resolve_tie(const WalkArgs &a, long long pl, int i, char *wave_l
         }
     }
     if (lane == 0) {
-        const long long offA = ((long long)a.slot_of[na] * n + i) * a.rs;
-        const long long offB = ((long long)a.slot_of[nb] * n + i) * a.rs;
-        const unsigned rx = a.rank[offA + jx], ry = a.rank[offB + jy];
+        const long long offA = table_row(a.slot_of[na], i, n, a.rs);
+        const long long offB = table_row(a.slot_of[nb], i, n, a.rs);
+        const unsigned rx = a.rank[table_at(offA, jx)], ry = a.rank[table_at(offB, jy)];
         a.cnt[pl * n + i] = (rx - 1u) | ((ry - 1u) << 16);
     }
     ksg::wave_sync();
