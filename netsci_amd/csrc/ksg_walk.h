@@ -137,12 +137,13 @@ inline size_t walk_lds_bytes(int d, int n, int nbuf) {
     return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 5 * nbuf) * sizeof(int);
 }
 
-// Table layout: the rows of 16 consecutive frames of a node are stored
-// block-interleaved, [slot][frame / 16][position / 8][frame % 16][8], so the
+// Table layout: the rows of 8 consecutive frames of a node are stored
+// block-interleaved, [slot][frame / 8][position / 8][frame % 8][8], so the
 // lanes that walk consecutive frames of one pair side by side read the same
-// block of their rows from two adjacent 128-byte lines. table_row gives the
+// block of their rows from one 128-byte line (groups of 16, 32 and 64 rows
+// walk as fast; 8 keeps the build's writes densest). table_row gives the
 // offset of (slot, frame i)'s position 0, table_at that of position e.
-constexpr int kRowGroup = 16;
+constexpr int kRowGroup = 8;
 NSX_HD inline long long table_rows(int n) {
     return ((long long)n + kRowGroup - 1) / kRowGroup * kRowGroup;
 }
