@@ -44,8 +44,8 @@ HBM_PEAK = 8.0e12  # B/s
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nodes", type=int, default=1000)
     ap.add_argument("--frames", type=int, default=2000)
     ap.add_argument("--k", type=int, default=6)
@@ -154,24 +154,34 @@ def main():
     timer = abi.Timer()
 
     def step():
+        t = time.perf_counter()
         timer.start()
         abi.ksg_mi(dX, X.shape[1], n, d, n, dab, P, k, dmi)
         timer.stop()
         mi = dmi.numpy()
         gc = abi.gc_from_mi(mi, d)
-        return mi, gc, timer.ms()
+        return mi, gc, timer.ms(), 1e3 * (time.perf_counter() - t)
+
+    def phase_line(i, kms, sms):
+        ph = abi.last_phase_ms() if abi.last_method()[0] == 2 else {}
+        parts = " ".join(f"{key} {v:.1f}" for key, v in ph.items())
+        log(f"step {i}: {sms:.1f} ms (call {abi.last_call_ms():.1f}, stream {kms:.1f}; "
+            f"{parts})")
+        return ph
 
     print(f"bench: {dev_name}, {cus} CUs; {P} pairs, n={n}", file=sys.stderr, flush=True)
     for w in range(args.warmup):
-        step()
-        print(f"bench: warmup {w} done", file=sys.stderr, flush=True)
+        _, _, kms, sms = step()
+        phase_line(f"warmup {w}", kms, sms)
     abi.sync()
-    kernel_ms = []
+    kernel_ms, step_ms, step_phases, call_ms = [], [], [], []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        mi, gc, kms = step()
+    for i in range(args.steps):
+        mi, gc, kms, sms = step()
         kernel_ms.append(kms)
-        print(f"bench: step kernel {kms:.1f} ms", file=sys.stderr, flush=True)
+        step_ms.append(sms)
+        call_ms.append(abi.last_call_ms())
+        step_phases.append(phase_line(i, kms, sms))
     abi.sync()
     wall = time.perf_counter() - t0
     ties = abi.last_tie_frames()
@@ -179,12 +189,15 @@ def main():
 
     ms_per_step = 1e3 * wall / args.steps
     value = P * args.steps / wall
-    kms = float(np.mean(kernel_ms))
+    kms = float(np.median(kernel_ms))
     # The reference algorithm's work (every ordered frame pair, both passes):
     # reported beside the roofline, never folded into it.
     work_per_pair = (8 * d + 2) * n * (n - 1)
     equivalent = work_per_pair * P / (kms * 1e-3)
-    phases = abi.last_phase_ms() if method == 2 else None
+    # Median over the timed steps of every phase (each step's phases are in
+    # steps_ms below).
+    phases = ({key: float(np.median([p[key] for p in step_phases])) for key in step_phases[0]}
+              if method == 2 else None)
     if method == 2:
         # Sorted-list path: the walk kernel dominates. Its algorithmic work
         # per list entry: both marginal distances (2d subtractions, 2(d-1)
@@ -234,7 +247,7 @@ def main():
 
     screened = None
     if not args.no_screen:
-        screened = screened_at_workload(abi, X, ab, N, n, k, dX, gc, 1e3 * wall / args.steps)
+        screened = screened_at_workload(abi, X, ab, N, n, k, dX, gc, float(np.median(step_ms)))
 
     out = {
         "metric": METRIC,
@@ -260,6 +273,10 @@ def main():
         "list_entries": list_entries,
         "phases_ms": phases,
         "call_ms": kms,
+        "steps_ms": {"step": [round(v, 2) for v in step_ms],
+                     "call": [round(v, 2) for v in call_ms],
+                     "median": float(np.median(step_ms)),
+                     "max_over_median": float(np.max(step_ms) / np.median(step_ms))},
         "all_frames_equivalent": {
             "rate": equivalent / 1e12, "unit": "T lane-ops/s",
             "multiple_of_valu_peak": equivalent / VALU_PEAK,
@@ -276,7 +293,8 @@ def screened_at_workload(abi, X, ab, N, n, k, dX, plain_gc, plain_ms):
     seeded trajectories of the same size (seeds 100-107), then every pair of
     this X through it (features, MLP, selection, exact KSG on candidates plus
     a 1% audit). Outside the plain pass's timed region; one warm-up pass,
-    then one timed pass. Recall is against the plain pass's exact GC."""
+    then one timed pass. Recall is against the plain pass's exact GC; the
+    speedup divides the median plain step by the screened pass."""
     from netsci_amd import screen
     t0 = time.perf_counter()
     feat, y = screen.training_set(trajectories=8, N=N, n=n, pairs_per=60000, k=k, seed=100)

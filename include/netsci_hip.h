@@ -111,9 +111,10 @@ int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symme
 
 /* Algorithm of nsx_ksg_mi and nsx_ksg_mi_windows (each window on its own
  * tables, one after another): method 0 = the sorted-list path
- * whenever it applies (at most 8192 frames whose two rows fit in LDS, finite coordinates, its tables
- * of 4*n*n bytes per used node fit in free device memory) and the all-frames
- * kernel otherwise; 1 = always the all-frames kernel; 2 = the list path or
+ * whenever it applies (at most 8192 frames whose two rows fit in LDS, finite
+ * coordinates, room for the tables of 4*n*n bytes of at least two nodes:
+ * when all used nodes do not fit, they are built block by block) and the
+ * all-frames kernel otherwise; 1 = always the all-frames kernel; 2 = the list path or
  * an error. walk_waves (1..16) and walk_batch (1..64) shape the list path's
  * walk kernel (0 = default). Results are identical for every choice. No
  * reference counterpart (the reference always compares all frame pairs,
@@ -130,6 +131,31 @@ int nsx_ksg_last_method(int *method, long long *list_entries);
  * [2] tie resolution, [3] frame-order psi sums. A diagnostic for roofline
  * accounting; no reference counterpart. */
 int nsx_ksg_last_phase_ms(double *phase_ms);
+
+/* Host wall time (ms) of the last nsx_ksg_mi / nsx_ksg_mi_windows call on
+ * this thread, from entry to return (it ends with a stream synchronisation),
+ * i.e. the device phases above plus the host's own work. No reference
+ * counterpart. */
+int nsx_ksg_last_call_ms(double *wall_ms);
+
+/* The list path's table layout in the last call on this thread: blocks = 1
+ * when every used node's tables fit at once, else the node blocks of the
+ * blocked schedule (one block resident while the others stream through the
+ * second half of the tables); slots = table slots held; node_builds = node
+ * tables built, summed over windows. Zeros when the call did not take the
+ * list path. No reference counterpart. */
+int nsx_ksg_last_list_info(int *blocks, long long *slots, long long *node_builds);
+
+/* Cap (bytes) on the device memory the list path's tables may use; 0 = the
+ * default, 85% of free device memory less 1 GiB. The tables are kept between
+ * calls and grow only when a call needs more; a cap below what is held
+ * applies from the next call. No reference counterpart. */
+int nsx_ksg_set_table_budget(long long bytes);
+
+/* Frees the library's device workspaces and tables (synchronises the
+ * device); the next call allocates what it needs again. No reference
+ * counterpart. */
+int nsx_release_workspace(void);
 
 /* ---- batched plans: one hipGraph launch per replay ----------------------
  * A plan fixes X, the windows, the pairs and the output of

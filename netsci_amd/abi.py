@@ -42,6 +42,10 @@ SIGNATURES = {
     "nsx_ksg_set_method": [_I, _I, _I],
     "nsx_ksg_last_method": [_IP, _LLP],
     "nsx_ksg_last_phase_ms": [ctypes.POINTER(ctypes.c_double)],
+    "nsx_ksg_last_call_ms": [ctypes.POINTER(ctypes.c_double)],
+    "nsx_ksg_last_list_info": [_IP, _LLP, _LLP],
+    "nsx_ksg_set_table_budget": [_LL],
+    "nsx_release_workspace": [],
     "nsx_ksg_plan_create": [ctypes.POINTER(_VP), _VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I,
                             _VP, _I],
     "nsx_ksg_plan_launch": [_VP, _VP],
@@ -242,6 +246,31 @@ def last_phase_ms():
     out = (ctypes.c_double * 4)()
     lib().nsx_ksg_last_phase_ms(out)
     return {"build": out[0], "walk": out[1], "ties": out[2], "psi_sums": out[3]}
+
+
+def last_call_ms():
+    """Host wall time (ms) of the last ksg_mi / ksg_mi_windows call."""
+    v = ctypes.c_double(0)
+    lib().nsx_ksg_last_call_ms(ctypes.byref(v))
+    return v.value
+
+
+def last_list_info():
+    """(node blocks, table slots, node tables built) of the last list-path call;
+    blocks == 1 when every used node's tables fit at once."""
+    b, s, nb = ctypes.c_int(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+    lib().nsx_ksg_last_list_info(ctypes.byref(b), ctypes.byref(s), ctypes.byref(nb))
+    return b.value, s.value, nb.value
+
+
+def set_table_budget(nbytes=0):
+    """Cap on the device memory of the list path's tables (0 = default)."""
+    check(lib().nsx_ksg_set_table_budget(int(nbytes)), "set table budget")
+
+
+def release_workspace():
+    """Free the library's device workspaces and tables."""
+    check(lib().nsx_release_workspace(), "release workspace")
 
 
 def last_tie_frames():

@@ -26,6 +26,15 @@ struct DeviceBuffer {
     T *as() const { return (T *)ptr; }
 };
 
+// The reference's GPU driver has kernels only for xd == 2 (two variables)
+// and leaves its kernel pointer uninitialised otherwise
+// (mutual_information_gpu.cu:1015-1030); its CPU path ignores xd
+// (mutual_information_cpu.cpp:6-121), and so does the CPU platform here.
+void validate_xd(int xd, bool gpu) {
+    if (gpu && xd != 2)
+        throw std::invalid_argument("xd must be 2 on the GPU platform (MI of two variables)");
+}
+
 void validate_pairs(CuArray<float> *X, CuArray<int> *ab, int k, int n, int d) {
     if (!X || !ab) throw std::invalid_argument("X and ab must be CuArrays");
     if (d < 1) throw std::invalid_argument("d must be >= 1");
@@ -66,12 +75,13 @@ void to_gc(float *v, long long count, int d) {
 }
 
 // The pair loop of mutual_information.cpp:103-172 / generalized_correlation.cpp:104-173.
-int pair_loop(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n, int d,
-              int platform, bool gc) {
+int pair_loop(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n, int xd,
+              int d, int platform, bool gc) {
     R->init(1, ab->m());
     if (ab->m() == 0) return platform;
     if (platform != GPU_PLATFORM && platform != CPU_PLATFORM)
         throw std::runtime_error("Invalid platform");
+    validate_xd(xd, platform == GPU_PLATFORM);
     validate_pairs(X, ab, k, n, d);
     if (platform == GPU_PLATFORM)
         gpu_mi(X->host(), X->m(), X->n(), n, ab->host(), ab->m(), k, n, d, R->host());
@@ -85,8 +95,8 @@ int pair_loop(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int
 // as the reference — <base>_ab.npy, and after pair i with i % freq == 0 the
 // whole R (zeros after i) as <base>_<i>.npy, removing <base>_<i-freq>.npy.
 // Pairs run in batches that end exactly at those checkpoints.
-int pair_loop_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n, int d,
-                   int platform, bool gc, int freq, std::string name) {
+int pair_loop_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n, int xd,
+                   int d, int platform, bool gc, int freq, std::string name) {
     if (freq <= 0)
         throw std::invalid_argument("checkpointFrequency must be >= 1 (the reference's -1 "
                                     "writes a file per pair and 0 divides by zero)");
@@ -97,6 +107,7 @@ int pair_loop_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k
     if (ab->m() == 0) return platform;
     if (platform != GPU_PLATFORM && platform != CPU_PLATFORM)
         throw std::runtime_error("Invalid platform");
+    validate_xd(xd, platform == GPU_PLATFORM);
     validate_pairs(X, ab, k, n, d);
     const long long P = ab->m();
     long long start = 0;
@@ -122,8 +133,10 @@ int pair_loop_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k
 }
 
 // One pair from two flat arrays (Xa, Xb hold d*n values at indices i + di*n).
-float single_pair(CuArray<float> *Xa, CuArray<float> *Xb, int k, int n, int d, bool gpu) {
+float single_pair(CuArray<float> *Xa, CuArray<float> *Xb, int k, int n, int xd, int d,
+                  bool gpu) {
     if (!Xa || !Xb) throw std::invalid_argument("Xa and Xb must be CuArrays");
+    validate_xd(xd, gpu);
     if (d < 1) throw std::invalid_argument("d must be >= 1");
     if (k < 1 || k >= n) throw std::invalid_argument("need 1 <= k < n");
     const long long len = (long long)d * n;
@@ -149,39 +162,33 @@ float gc_of(float mi, int d) {
 
 int mutualInformation(CuArray<float> *X, CuArray<float> *I, CuArray<int> *ab, int k, int n,
                       int xd, int d, int platform) {
-    (void)xd;
-    return pair_loop(X, I, ab, k, n, d, platform, false);
+    return pair_loop(X, I, ab, k, n, xd, d, platform, false);
 }
 
 int mutualInformation(CuArray<float> *X, CuArray<float> *I, CuArray<int> *ab, int k, int n,
                       int xd, int d, int platform, int checkpointFrequency,
                       std::string checkpointFileName) {
-    (void)xd;
-    return pair_loop_ckpt(X, I, ab, k, n, d, platform, false, checkpointFrequency,
+    return pair_loop_ckpt(X, I, ab, k, n, xd, d, platform, false, checkpointFrequency,
                           checkpointFileName);
 }
 
 float mutualInformationGpu(CuArray<float> *Xa, CuArray<float> *Xb, int k, int n, int xd, int d) {
-    (void)xd;
-    return single_pair(Xa, Xb, k, n, d, true);
+    return single_pair(Xa, Xb, k, n, xd, d, true);
 }
 
 float mutualInformationCpu(CuArray<float> *Xa, CuArray<float> *Xb, int k, int n, int xd, int d) {
-    (void)xd;
-    return single_pair(Xa, Xb, k, n, d, false);
+    return single_pair(Xa, Xb, k, n, xd, d, false);
 }
 
 int generalizedCorrelation(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n,
                            int xd, int d, int platform) {
-    (void)xd;
-    return pair_loop(X, R, ab, k, n, d, platform, true);
+    return pair_loop(X, R, ab, k, n, xd, d, platform, true);
 }
 
 int generalizedCorrelation(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n,
                            int xd, int d, int platform, int checkpointFrequency,
                            std::string checkpointFileName) {
-    (void)xd;
-    return pair_loop_ckpt(X, R, ab, k, n, d, platform, true, checkpointFrequency,
+    return pair_loop_ckpt(X, R, ab, k, n, xd, d, platform, true, checkpointFrequency,
                           checkpointFileName);
 }
 
@@ -401,6 +408,17 @@ void standardDeviation(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sig
     } else {
         throw std::invalid_argument("Invalid platform");
     }
+}
+
+void recoverSingleShortestPath(int **NUMPY_ARRAY, int **NUMPY_ARRAY_DIM1, CuArray<int> *paths,
+                               int maxPathLength, int i, int j) {
+    if (!NUMPY_ARRAY || !NUMPY_ARRAY_DIM1)
+        throw std::invalid_argument("recoverSingleShortestPath: null output pointers");
+    const std::vector<int> path = recoverSingleShortestPath(paths, maxPathLength, i, j);
+    *NUMPY_ARRAY_DIM1 = new int[1];
+    (*NUMPY_ARRAY_DIM1)[0] = (int)path.size();
+    *NUMPY_ARRAY = new int[path.size()];
+    std::copy(path.begin(), path.end(), *NUMPY_ARRAY);
 }
 
 std::vector<int> recoverSingleShortestPath(CuArray<int> *paths, int maxPathLength, int i, int j) {

@@ -65,7 +65,13 @@ void standardDeviation(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sig
                        int platform);
 void standardDeviationGpu(CuArray<float> *a, CuArray<float> *u, CuArray<float> *sigma, int m,
                           int n);
-// hedetniemi.cpp:71-108: the path i -> j as node indices.
+// hedetniemi.h:173-180 / hedetniemi.cpp:71-108, the reference's signature:
+// the path i -> j as node indices in a new int[] of (*NUMPY_ARRAY_DIM1)[0]
+// entries; *NUMPY_ARRAY_DIM1 is a new int[1]. Both are new[]-allocated, as
+// the reference allocates them, and the caller releases them with delete[].
+void recoverSingleShortestPath(int **NUMPY_ARRAY, int **NUMPY_ARRAY_DIM1, CuArray<int> *paths,
+                               int maxPathLength, int i, int j);
+// The same path as a vector (what the Python binding returns as an int32 array).
 std::vector<int> recoverSingleShortestPath(CuArray<int> *paths, int maxPathLength, int i, int j);
 
 // Bootstrap replicates as frame windows (no reference counterpart; each

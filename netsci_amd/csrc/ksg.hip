@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -65,7 +66,7 @@ int make_problem(Problem &pb, const float *X, long long ld, long long dstride, i
                  const int *off, const int *nfr, int R, const int *ab, long long P, int k) {
     NSX_REQUIRE(X && ab, "ksg: null pointer argument");
     NSX_REQUIRE(d >= 1 && d <= 3, "ksg: the GPU path supports d = 1, 2 or 3");
-    NSX_REQUIRE(k >= 1 && k <= ksg::kMaxK, "ksg: the GPU path supports 1 <= k <= 16");
+    NSX_REQUIRE(k >= 1, "ksg: k must be >= 1");
     NSX_REQUIRE(P >= 0 && R >= 1, "ksg: bad pair or window count");
     NSX_REQUIRE(ld > 0 && dstride > 0, "ksg: bad strides");
     NSX_REQUIRE((long long)(d - 1) * dstride + dstride <= ld,
@@ -79,6 +80,8 @@ int make_problem(Problem &pb, const float *X, long long ld, long long dstride, i
         nmax = std::max(nmax, nfr[r]);
         work += (double)nfr[r] * (nfr[r] - 1) * (double)P;
     }
+    NSX_REQUIRE(k <= ksg::kMaxK || nmax <= ksgw::kMaxFrames,
+                "ksg: k > 16 needs windows of at most 8192 frames");
     // Longest windows first, so the biggest items start first.
     std::vector<int> order(R);
     std::iota(order.begin(), order.end(), 0);
@@ -288,6 +291,7 @@ thread_local long long g_last_entries = 0;  // list entries walked (both lists)
 // Device time of the last list-path call's phases (hipEvents on its stream):
 // [0] node map + table build, [1] walk, [2] tie resolution, [3] psi sums.
 thread_local double g_last_phase_ms[4] = {0, 0, 0, 0};
+thread_local double g_last_call_ms = 0;  // host wall time of the last call
 
 // Event pairs for phase timing, created once per thread.
 struct PhaseEvents {
@@ -390,18 +394,25 @@ __global__ __launch_bounds__(1024) void slot_scan_kernel(const int *used, int N,
     if (threadIdx.x == 0) *count = carry;
 }
 
-// span[s] = max over dimensions d of (max - min) of coordinate d of node
-// node_of[s] over the window's n frames: an upper bound of every marginal
-// distance of the node (rounding of the differences is monotone). One wave
-// per node.
-__global__ __launch_bounds__(64) void span_kernel(const float *X, long long ld, long long dstride,
-                                                  int d, int n, const int *node_of, float *span) {
-    const float *x = X + (long long)node_of[blockIdx.x] * ld;
+// One wave per used node (blocks past *count exit): span[node] = max over
+// dimensions of (max - min) of the coordinate over the window's n frames,
+// an upper bound of every marginal distance of the node (rounding of the
+// differences is monotone); *bad |= 1 if any of those coordinates is not
+// finite.
+__global__ __launch_bounds__(64) void node_stats_kernel(const float *X, long long ld,
+                                                        long long dstride, int d, int n,
+                                                        const int *node_of, const int *count,
+                                                        float *span, unsigned *bad) {
+    if ((int)blockIdx.x >= *count) return;
+    const int node = node_of[blockIdx.x];
+    const float *x = X + (long long)node * ld;
     float best = 0.0f;
+    bool nonfinite = false;
     for (int t = 0; t < d; ++t) {
         float lo = __builtin_inff(), hi = -__builtin_inff();
         for (int f = threadIdx.x; f < n; f += 64) {
             const float v = x[t * dstride + f];
+            nonfinite |= !__builtin_isfinite(v);
             lo = fminf(lo, v);
             hi = fmaxf(hi, v);
         }
@@ -412,7 +423,40 @@ __global__ __launch_bounds__(64) void span_kernel(const float *X, long long ld, 
         }
         best = fmaxf(best, hi - lo);
     }
-    if (threadIdx.x == 0) span[blockIdx.x] = best;
+    if (threadIdx.x == 0) span[node] = best;
+    if (__ballot(nonfinite) && threadIdx.x == 0) atomicOr(bad, 1u);
+}
+
+// Node-blocked tables: the block pair of every pair (blocks of bn slots,
+// (lo, hi) with lo <= hi), as its position in the schedule, and a histogram.
+__global__ void pair_key_kernel(const int *ab, long long P, const int *slot_of, int bn, int nb,
+                                const int *sched_of, int *key, unsigned *hist) {
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (long long)gridDim.x * blockDim.x) {
+        const int x = slot_of[ab[2 * p]] / bn, y = slot_of[ab[2 * p + 1]] / bn;
+        const int kk = sched_of[min(x, y) * nb + max(x, y)];
+        key[p] = kk;
+        atomicAdd(hist + kk, 1u);
+    }
+}
+
+// Scatter the pairs into schedule order (the order within one block pair is
+// arbitrary: every pair's result is independent of it).
+__global__ void pair_scatter_kernel(const int *ab, long long P, const int *key, unsigned *cursor,
+                                    int *ab_perm, int *idx_perm) {
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P;
+         p += (long long)gridDim.x * blockDim.x) {
+        const unsigned at = atomicAdd(cursor + key[p], 1u);
+        ab_perm[2 * (long long)at] = ab[2 * p];
+        ab_perm[2 * (long long)at + 1] = ab[2 * p + 1];
+        idx_perm[at] = (int)p;
+    }
+}
+
+// Resident table slot of the nodes of one block: rslot_of[node_of[s]] = base + s.
+__global__ void set_slots_kernel(const int *node_of, int count, int base, int *rslot_of) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < count) rslot_of[node_of[s]] = base + s;
 }
 
 // psi terms of every frame summed sequentially in frame order, then the MI
@@ -423,11 +467,15 @@ __global__ __launch_bounds__(64) void span_kernel(const float *X, long long ld, 
 constexpr int kSumWaves = 4;
 __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a) {
     __shared__ float tile[kSumWaves][64][65];
+    if (a.abort && *a.abort) return;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x / 64;
     const long long p0 = ((long long)blockIdx.x * kSumWaves + w) * 64;
     if (p0 >= a.P) return;
     const int rows = (int)min(64LL, a.P - p0);
+    // Output index of this lane's pair, and of row r's pair (for counts).
+    const long long mine = a.pbase + p0 + lane;
+    const long long out_idx = lane < rows ? (a.perm ? (long long)a.perm[mine] : mine) : 0;
     float acc = 0.0f;
     for (int f0 = 0; f0 < a.n; f0 += 64) {
         const int f = f0 + lane;
@@ -444,12 +492,17 @@ __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a
             for (int u = 0; u < 16; ++u) {
                 const int r = r0 + u;
                 if (r < rows) {
+                    // Row r's output index (all lanes take part in the shuffle).
+                    long long pr = 0;
+                    if (a.counts_out)
+                        pr = (long long)(unsigned)__shfl((int)(unsigned)out_idx, r) |
+                             ((long long)__shfl((int)(out_idx >> 32), r) << 32);
                     float term = 0.0f;
                     if (f < a.n) {
                         const int nx = (int)(v[u] & 0xffffu), ny = (int)(v[u] >> 16);
                         term = a.psi[nx] + a.psi[ny];
                         if (a.counts_out) {
-                            const long long at = (p0 + r) * a.n + f;
+                            const long long at = pr * a.n + f;
                             a.counts_out[at * 2 + 0] = nx;
                             a.counts_out[at * 2 + 1] = ny;
                         }
@@ -465,43 +518,146 @@ __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a
     }
     if (lane < rows) {
         const float avg = acc / (float)a.n;
-        a.mi_out[p0 + lane] = a.psi[a.n] + a.psi[a.k] - a.inv_k - avg;
+        a.mi_out[out_idx] = a.psi[a.n] + a.psi[a.k] - a.inv_k - avg;
     }
 }
 
-// Runs the whole problem on the list path when it applies (windows of at
-// most ksgw::kMaxFrames frames whose pair records fit LDS, tables within the
-// free device memory, finite coordinates); otherwise returns kOk with
-// *ran == false. Windows run one after another, each with its own tables
-// (the sorted lists depend on the frame set), into mi_out[rep * P + p].
-int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_out,
-              hipStream_t stream, bool *ran) {
-    *ran = false;
-    if (g_method == 1) return kOk;
-    const size_t lds_cap = 160 * 1024;
-    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
-    const bool eligible = pb.nmax <= ksgw::kMaxFrames && rec_bytes + 256 <= lds_cap &&
-                          (pb.R == 1 || counts_out == nullptr);
-    if (!eligible) {
-        NSX_REQUIRE(g_method != 2,
-                    "ksg: the list path needs windows of at most 8192 frames whose two rows "
-                    "fit in LDS (2*d*n floats <= 160 KiB)");
+// ---- list path: layout, schedule and launches ------------------------------
+//
+// Tables hold S node slots. When every used node fits (S >= U) one build
+// covers them all and one walk covers all pairs in their own order. Otherwise
+// the used nodes (in node order) form nb blocks of bn = S / 2 slots, the
+// pairs are grouped by block pair on the device, and the schedule keeps one
+// block resident as the "row" while the blocks after it stream through the
+// other half of the tables, last to first, so that the next row block is
+// already resident when its turn comes: 1 + nb (nb - 1) / 2 block builds.
+
+struct ListOp {
+    bool build;       // build block `block` into half `half`; else walk
+    int block, half;
+    long long p0, pc; // walk: pairs [p0, p0 + pc) of the (permuted) pair list
+};
+
+struct ListLayout {
+    int N = 0, U = 0;
+    long long S = 0;       // resident table slots
+    int nb = 1, bn = 0;    // blocks and slots per block
+    bool tables = true;    // false: k > kMaxK on the all-frames method (counts by a pass)
+    std::vector<ListOp> ops;
+    long long node_builds = 0;  // per window
+    // Device buffers.
+    int *used = nullptr, *slot_of = nullptr, *node_of = nullptr, *rslot_of = nullptr,
+        *count = nullptr;
+    float *span = nullptr;  // [R][N]
+    unsigned *flags = nullptr;  // [0] non-finite flag, [8..11] walk stats (2 x u64),
+                                // [16] tie-list count
+    const int *ab = nullptr;    // pair list the walks index (pb.ab or ab_perm)
+    const int *perm = nullptr;  // nullable: output index of each listed pair
+    uint16_t *ord = nullptr, *rank = nullptr;
+    unsigned *cnt = nullptr, *tie_list = nullptr;
+    long long chunk = 0;   // pairs per walk launch (packed counts below 4 GiB)
+    unsigned tie_cap = 0;
+};
+
+// Fixed launch shape of the walk for this problem.
+struct WalkShape {
+    ksgw::WalkFn walk = nullptr;
+    ksgw::TieFn tiek = nullptr, tieo = nullptr;
+    int wpb = 16;
+    long long grid_max = 0;
+};
+
+ksgw::WalkFn bigk_for(int d) {
+    switch (d) {
+        case 1: return ksgw::bigk_d1();
+        case 2: return ksgw::bigk_d2();
+        case 3: return ksgw::bigk_d3();
+        default: return nullptr;
+    }
+}
+
+int walk_shape(const Problem &pb, WalkShape &ws) {
+    if (pb.k > ksg::kMaxK) {
+        // One wave per query (ksgw::bigk_kernel); as many waves per CU as
+        // their LDS rows allow, a few rounds of them.
+        ws.walk = bigk_for(pb.d);
+        NSX_REQUIRE(ws.walk, "ksg: no kernel for (d, k)");
+        ws.wpb = 1;
+        const size_t per = ksgw::bigk_lds_bytes(pb.nmax);
+        ws.grid_max = (long long)cu_count() * std::max<long long>(1, std::min<long long>(16, (160 * 1024) / per));
         return kOk;
     }
-    const ksgw::WalkFn walk = walk_for(pb.d, pb.k);
-    const ksgw::TieFn tiek = tie_for(pb.d, pb.k, false);
-    const ksgw::TieFn tieo = tie_for(pb.d, pb.k, true);
-    NSX_REQUIRE(walk && tiek && tieo, "ksg: no list-path kernel for (d, k)");
-    const long long m2 = 2 * pb.P;
-    for (int t = 0; t < 4; ++t) g_last_phase_ms[t] = 0;
-    hipEvent_t ev_start = g_phase_events.get(0);
-    if (ev_start) (void)hipEventRecord(ev_start, stream);
+    ws.walk = walk_for(pb.d, pb.k);
+    ws.tiek = tie_for(pb.d, pb.k, false);
+    ws.tieo = tie_for(pb.d, pb.k, true);
+    NSX_REQUIRE(ws.walk && ws.tiek && ws.tieo, "ksg: no list-path kernel for (d, k)");
+    const size_t lds_cap = 160 * 1024;
+    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    ws.wpb = std::min(g_walk_wpb, ksgw::walk_max_waves(pb.k));
+    int per_cu = 0;
+    const int nbuf_max = (int)std::min<size_t>(3, (lds_cap - 256) / rec_bytes);
+    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void *)ws.walk, ksgw::kWave * ws.wpb,
+                ksgw::walk_lds_bytes(pb.d, pb.nmax, nbuf_max)),
+            "walk occupancy");
+    ws.grid_max = (long long)cu_count() * std::max(per_cu, 1);
+    return kOk;
+}
 
-    // Used nodes and their table slots.
-    // mm: [0] min node, [1] max node, [4] non-finite flag, [8..11] walk
-    // stats (2 x u64), [16] tie-list count.
-    int *mm = (int *)workspace(kWsListStats, 256);
-    if (!mm) return kHipError;
+// Bytes of one node slot of the tables (ord and rank) for windows of up to
+// nmax frames.
+size_t slot_bytes_for(int nmax) {
+    const long long rs = ((long long)nmax + 7) & ~7LL;
+    return (size_t)2 * (size_t)ksgw::table_rows(nmax) * (size_t)rs * sizeof(uint16_t);
+}
+
+// The schedule for U used nodes in S slots (see above). sched_of[x * nb + y]
+// (x <= y) is the position of block pair (x, y) among the walks.
+void make_schedule(ListLayout &L, std::vector<int> &sched_of) {
+    L.ops.clear();
+    L.node_builds = 0;
+    if (L.S >= L.U) {
+        L.nb = 1;
+        L.bn = L.U;
+        L.ops.push_back({true, 0, 0, 0, 0});
+        L.ops.push_back({false, 0, 0, 0, 0});
+        L.node_builds = L.U;
+        sched_of.assign(1, 0);
+        return;
+    }
+    L.bn = (int)(L.S / 2);
+    L.nb = (L.U + L.bn - 1) / L.bn;
+    const int nb = L.nb;
+    auto bsize = [&](int b) { return std::min(L.bn, L.U - b * L.bn); };
+    sched_of.assign((size_t)nb * nb, -1);
+    int walks = 0, h = 0;
+    L.ops.push_back({true, 0, h, 0, 0});
+    L.node_builds += bsize(0);
+    for (int I = 0; I < nb; ++I) {
+        sched_of[I * nb + I] = walks++;
+        L.ops.push_back({false, I * nb + I, h, 0, 0});
+        for (int J = nb - 1; J > I; --J) {
+            L.ops.push_back({true, J, 1 - h, 0, 0});
+            L.node_builds += bsize(J);
+            sched_of[I * nb + J] = walks++;
+            L.ops.push_back({false, I * nb + J, h, 0, 0});
+        }
+        h = 1 - h;
+    }
+}
+
+// Host-side sizing with device readbacks (not capturable): the node map,
+// every window's spans and the finite check, the table slots, the
+// schedule, and for blocked tables the pair permutation. *ok = false: the
+// list path does not apply (non-finite coordinates or no room for tables).
+int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok) {
+    *ok = false;
+    const long long m2 = 2 * pb.P;
+    // flags: [0] min node, [1] max node (first), then reused as described
+    // in ListLayout.
+    unsigned *flags = (unsigned *)workspace(kWsListStats, 256);
+    if (!flags) return kHipError;
+    int *mm = (int *)flags;
     int rc = fill_u32(mm, 0x7fffffffu, 1, stream);
     if (!rc) rc = fill_u32(mm + 1, 0xffffffffu, 1, stream);
     if (rc) return rc;
@@ -515,216 +671,387 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     NSX_HIP(hipStreamSynchronize(stream), "ab range sync");
     NSX_REQUIRE(h_mm[0] >= 0, "ksg: negative node index in ab");
     const int N = h_mm[1] + 1;
+    L.N = N;
     int *map = (int *)workspace(kWsListMap, ((size_t)4 * N + 16) * sizeof(int));
-    if (!map) return kHipError;
-    int *used = map, *slot_of = map + N, *node_of = map + 2 * N, *d_count = map + 3 * N;
-    float *span = (float *)(map + 3 * N + 16);  // per table slot
-    rc = fill_u32(used, 0u, (size_t)N, stream);
+    L.span = (float *)workspace(kWsListSpan, (size_t)pb.R * N * sizeof(float));
+    if (!map || !L.span) return kHipError;
+    L.used = map;
+    L.slot_of = map + N;
+    L.node_of = map + 2 * N;
+    L.rslot_of = map + 3 * N;
+    L.count = map + 4 * N;
+    L.flags = flags;
+    rc = fill_u32(flags, 0u, 32, stream);
+    if (!rc) rc = fill_u32(L.used, 0u, (size_t)N, stream);
     if (rc) return rc;
     hipLaunchKernelGGL(mark_nodes_kernel, dim3((unsigned)rblocks), dim3(256), 0, stream, pb.ab,
-                       m2, used);
+                       m2, L.used);
     NSX_HIP(hipGetLastError(), "mark nodes launch");
-    hipLaunchKernelGGL(slot_scan_kernel, dim3(1), dim3(1024), 0, stream, used, N, slot_of,
-                       node_of, d_count);
+    hipLaunchKernelGGL(slot_scan_kernel, dim3(1), dim3(1024), 0, stream, L.used, N, L.slot_of,
+                       L.node_of, L.count);
     NSX_HIP(hipGetLastError(), "slot scan launch");
-    int U = 0;
-    NSX_HIP(hipMemcpyAsync(&U, d_count, sizeof(int), hipMemcpyDeviceToHost, stream),
+    for (int w = 0; w < pb.R; ++w) {
+        hipLaunchKernelGGL(node_stats_kernel, dim3((unsigned)N), dim3(64), 0, stream,
+                           pb.X + pb.host_win[w], pb.ld, pb.dstride, pb.d, pb.host_win[pb.R + w],
+                           L.node_of, L.count, L.span + (long long)w * N, flags);
+        NSX_HIP(hipGetLastError(), "node stats launch");
+    }
+    int h_u[2] = {0, 0};
+    NSX_HIP(hipMemcpyAsync(&h_u[0], L.count, sizeof(int), hipMemcpyDeviceToHost, stream),
             "slot count readback");
+    NSX_HIP(hipMemcpyAsync(&h_u[1], flags, sizeof(int), hipMemcpyDeviceToHost, stream),
+            "finite check readback");
     NSX_HIP(hipStreamSynchronize(stream), "slot scan sync");
-
-    // Tables (ord and rank, U x n rows of rs entries each), sized for the
-    // longest window and reused by every window.
-    const long long rs_max = ((long long)pb.nmax + 7) & ~7LL;
-    const size_t table_bytes =
-        (size_t)U * (size_t)ksgw::table_rows(pb.nmax) * rs_max * sizeof(uint16_t);
-    size_t free_b = 0, total_b = 0;
-    NSX_HIP(hipMemGetInfo(&free_b, &total_b), "memory info");
-    const double budget = 0.85 * (double)free_b - 512.0 * (1 << 20);
-    if ((double)(2 * table_bytes) > budget) {
-        NSX_REQUIRE(g_method != 2, "ksg: the list-path tables do not fit in device memory");
+    L.U = h_u[0];
+    if (h_u[1]) {
+        NSX_REQUIRE(g_method != 2, "ksg: the list path needs finite coordinates");
+        NSX_REQUIRE(pb.k <= ksg::kMaxK, "ksg: k > 16 needs finite coordinates");
         return kOk;
     }
-    uint16_t *tables = nullptr;
-    NSX_HIP(hipMalloc(&tables, 2 * table_bytes + 64), "list tables");
-    struct Free {
-        void *p;
-        ~Free() { (void)hipFree(p); }
-    } free_tables{tables};
-    uint16_t *ord = tables, *rank = tables + table_bytes / sizeof(uint16_t);
 
-    // Per-call buffers sized for the longest window.
-    const int wpb = std::min(g_walk_wpb, ksgw::walk_max_waves(pb.k));
-    int per_cu = 0;
-    const int nbuf_max = (int)std::min<size_t>(3, (lds_cap - 256) / rec_bytes);
-    NSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void *)walk, ksgw::kWave * wpb,
-                ksgw::walk_lds_bytes(pb.d, pb.nmax, nbuf_max)),
-            "walk occupancy");
-    per_cu = std::max(per_cu, 1);
-    const long long grid_max = (long long)cu_count() * per_cu;
-    // Packed counts of one launch: at most 4 GiB (larger problems launch in
-    // pair chunks), so local (pair, frame) indices stay below 2^30.
-    const long long chunk = std::max<long long>(
+    std::vector<int> sched_of;
+    if (pb.k > ksg::kMaxK && g_method == 1) {
+        // No tables: one walk over every pair, counts by a pass over the row.
+        L.tables = false;
+        L.S = L.U;
+        L.nb = 1;
+        L.bn = L.U;
+        L.ops.assign(1, ListOp{false, 0, 0, 0, 0});
+        sched_of.assign(1, 0);
+    } else {
+        // Table slots: all used nodes when they fit, else at least two.
+        void *tab = nullptr;
+        const size_t sb = slot_bytes_for(pb.nmax);
+        rc = table_space(sb, L.U, std::min(L.U, 2), &tab, &L.S);
+        if (rc) return rc;
+        if (L.S < std::min(L.U, 2)) {
+            NSX_REQUIRE(g_method != 2 && pb.k <= ksg::kMaxK,
+                        "ksg: no device memory for two nodes' list-path tables");
+            return kOk;
+        }
+        L.ord = (uint16_t *)tab;
+        L.rank = L.ord + L.S * (long long)(sb / 2 / sizeof(uint16_t));
+        make_schedule(L, sched_of);
+    }
+
+    // Pair order.
+    L.ab = pb.ab;
+    L.perm = nullptr;
+    if (L.nb == 1) {
+        L.ops.back().p0 = 0;
+        L.ops.back().pc = pb.P;
+    } else {
+        const int nb = L.nb;
+        const int nwalks = nb * (nb + 1) / 2;
+        const size_t perm_words = (size_t)pb.P * 4 + (size_t)nb * nb + 2 * (size_t)nwalks + 64;
+        int *pw = (int *)workspace(kWsListPerm, perm_words * sizeof(int));
+        if (!pw) return kHipError;
+        int *ab_perm = pw, *idx_perm = pw + 2 * pb.P, *key = pw + 3 * pb.P;
+        int *d_sched = pw + 4 * pb.P;
+        unsigned *hist = (unsigned *)(d_sched + (size_t)nb * nb);
+        unsigned *cursor = hist + nwalks;
+        NSX_HIP(hipMemcpyAsync(d_sched, sched_of.data(), sched_of.size() * sizeof(int),
+                               hipMemcpyHostToDevice, stream),
+                "schedule upload");
+        rc = fill_u32(hist, 0u, (size_t)nwalks, stream);
+        if (rc) return rc;
+        const long long pblocks = std::min<long long>((pb.P + 255) / 256, 4096);
+        hipLaunchKernelGGL(pair_key_kernel, dim3((unsigned)pblocks), dim3(256), 0, stream, pb.ab,
+                           pb.P, L.slot_of, L.bn, nb, d_sched, key, hist);
+        NSX_HIP(hipGetLastError(), "pair key launch");
+        std::vector<unsigned> h_hist(nwalks), h_cur(nwalks);
+        NSX_HIP(hipMemcpyAsync(h_hist.data(), hist, nwalks * sizeof(unsigned),
+                               hipMemcpyDeviceToHost, stream),
+                "histogram readback");
+        NSX_HIP(hipStreamSynchronize(stream), "histogram sync");
+        unsigned acc = 0;
+        for (int t = 0; t < nwalks; ++t) {
+            h_cur[t] = acc;
+            acc += h_hist[t];
+        }
+        NSX_HIP(hipMemcpyAsync(cursor, h_cur.data(), nwalks * sizeof(unsigned),
+                               hipMemcpyHostToDevice, stream),
+                "cursor upload");
+        hipLaunchKernelGGL(pair_scatter_kernel, dim3((unsigned)pblocks), dim3(256), 0, stream,
+                           pb.ab, pb.P, key, cursor, ab_perm, idx_perm);
+        NSX_HIP(hipGetLastError(), "pair scatter launch");
+        // The cursor upload reads h_cur: wait before it goes out of scope.
+        NSX_HIP(hipStreamSynchronize(stream), "pair scatter sync");
+        for (ListOp &op : L.ops) {
+            if (op.build) continue;
+            const int x = op.block / nb, y = op.block % nb;
+            const int t = sched_of[x * nb + y];
+            op.p0 = h_cur[t];
+            op.pc = h_hist[t];
+        }
+        // Drop the walks without pairs, and the column builds they needed
+        // (every column build is followed by its walk); the build of block
+        // I + 1 stays, as the next row block.
+        std::vector<ListOp> kept;
+        L.node_builds = 0;
+        for (size_t t = 0; t < L.ops.size(); ++t) {
+            const ListOp &op = L.ops[t];
+            if (op.build) {
+                const ListOp &w = L.ops[t + 1];
+                if (t > 0 && op.block != w.block / nb + 1 && w.pc == 0) {
+                    ++t;
+                    continue;
+                }
+                kept.push_back(op);
+                L.node_builds += std::min(L.bn, L.U - op.block * L.bn);
+            } else if (op.pc > 0) {
+                kept.push_back(op);
+            }
+        }
+        L.ops.swap(kept);
+        L.ab = ab_perm;
+        L.perm = idx_perm;
+    }
+
+    // Packed counts of one walk launch: at most 4 GiB, so local (pair,
+    // frame) indices stay below 2^30; the tie list holds 1/32 of a launch's
+    // queries (ties beyond it are marked in the counts and resolved by
+    // tie_overflow_kernel). One row of margin: a shorter window's chunk x n
+    // can pass chunk x nmax by less than nmax.
+    L.chunk = std::max<long long>(
         1, std::min<long long>(pb.P, (1LL << 32) / ((long long)pb.nmax * (long long)sizeof(unsigned))));
-    // Tie list: up to 1/32 of a chunk's queries (ties beyond it are marked
-    // in the counts and resolved by tie_overflow_kernel).
-    const unsigned tie_cap =
-        (unsigned)std::max<long long>(1024, chunk * (long long)pb.nmax / 32);
-    const long long tie_waves = (long long)cu_count() * 8;
-    // One row of margin: a shorter window's chunk x n can pass chunk x nmax
-    // by less than nmax.
-    const size_t cnt_words = (size_t)chunk * pb.nmax + pb.nmax;
-    unsigned *cnt = (unsigned *)workspace(kWsListCnt, (cnt_words + tie_cap) * sizeof(unsigned));
-    unsigned long long *stats = (unsigned long long *)(mm + 8);
-    unsigned *tie_count = (unsigned *)(mm + 16);
-    unsigned *bad = (unsigned *)(mm + 4);
-    if (!cnt) return kHipError;
-    unsigned *tie_list = cnt + cnt_words;
-    rc = fill_u32(stats, 0u, 4, stream);
-    if (rc) return rc;
+    L.tie_cap = (unsigned)std::max<long long>(1024, L.chunk * (long long)pb.nmax / 32);
+    const size_t cnt_words = (size_t)L.chunk * pb.nmax + pb.nmax;
+    L.cnt = (unsigned *)workspace(kWsListCnt, (cnt_words + L.tie_cap) * sizeof(unsigned));
+    if (!L.cnt) return kHipError;
+    L.tie_list = L.cnt + cnt_words;
+    *ok = true;
+    return kOk;
+}
 
-    size_t evi = 1;
+// Event pairs recorded around the phases of a list-path run.
+struct PhaseMarks {
     struct Mark {
         int phase;
         hipEvent_t a, b;
     };
-    std::vector<Mark> marks;  // (phase, start, stop) event pairs on the stream
+    std::vector<Mark> marks;
+    size_t next = 1;  // event 0 marks the call's start
+    bool on = false;
+    hipEvent_t ev() { return on ? g_phase_events.get(next++) : nullptr; }
+    void add(int phase, hipEvent_t a, hipEvent_t b) {
+        if (a && b) marks.push_back({phase, a, b});
+    }
+};
+
+// Enqueues every window's builds, walks, tie resolution and psi sums (no
+// host synchronisation; capturable once list_prepare has run).
+int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, const float *d_psi,
+                 float *mi_out, int *counts_out, hipStream_t stream, PhaseMarks *pm) {
+    const size_t lds_cap = 160 * 1024;
+    unsigned long long *stats = (unsigned long long *)(L.flags + 8);
+    unsigned *tie_count = L.flags + 16;
+    const unsigned *abort_flag = L.flags;
+    const long long tie_waves = (long long)cu_count() * 8;
     for (int w = 0; w < pb.R; ++w) {
         const int off = pb.host_win[w], n = pb.host_win[pb.R + w], rep = pb.host_win[2 * pb.R + w];
         const float *X = pb.X + off;
         const long long rs = ((long long)n + 7) & ~7LL;
+        const long long slot_entries = ksgw::table_rows(n) * rs;
         float *out = mi_out + (long long)rep * pb.P;
-
-        // (1) tables for this window's frames.
+        const float *span = L.span + (long long)w * L.N;
         const int width = ksgw::build_width(n);
         const ksgw::BuildFn buildk = build_for(pb.d, width);
         NSX_REQUIRE(buildk, "ksg: no list-path table kernel for this frame count");
-        rc = fill_u32(bad, 0u, 1, stream);
-        if (rc) return rc;
-        hipEvent_t b0 = g_phase_events.get(evi++), b1 = g_phase_events.get(evi++);
-        if (b0) (void)hipEventRecord(b0, stream);
-        ksgw::BuildArgs ba;
-        ba.X = X;
-        ba.ld = pb.ld;
-        ba.dstride = pb.dstride;
-        ba.n = n;
-        ba.rs = rs;
-        ba.node_of = node_of;
-        const int rows_per_pass = ksgw::kBuildWaves / ksgw::build_row_waves(width);
-        ba.groups = (int)std::max<long long>(
-            1, std::min<long long>((n + rows_per_pass - 1) / rows_per_pass,
-                                   (8LL * cu_count() + U - 1) / std::max(U, 1)));
-        ba.ord = ord;
-        ba.rank = rank;
-        ba.bad = bad;
-        hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)U * ba.groups)),
-                           dim3(ksgw::kWave * ksgw::kBuildWaves),
-                           ksgw::build_lds_bytes(pb.d, n, rs), stream, ba);
-        NSX_HIP(hipGetLastError(), "list build launch");
-        hipLaunchKernelGGL(span_kernel, dim3((unsigned)U), dim3(64), 0, stream, X, pb.ld,
-                           pb.dstride, pb.d, n, node_of, span);
-        NSX_HIP(hipGetLastError(), "span launch");
-        if (b1) (void)hipEventRecord(b1, stream);
-        marks.push_back({0, b0, b1});
-        if (w == 0) marks.push_back({0, ev_start, b0});  // the node map
-        unsigned h_bad = 0;
-        NSX_HIP(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, stream),
-                "finite check readback");
-        NSX_HIP(hipStreamSynchronize(stream), "list build sync");
-        if (h_bad) {
-            NSX_REQUIRE(g_method != 2, "ksg: the list path needs finite coordinates");
-            return kOk;
-        }
-
-        // (2) walk, ties, psi sums, per chunk of pairs.
         const size_t wrec = (size_t)2 * pb.d * n * sizeof(float);
         const int nbuf = (int)std::min<size_t>(3, (lds_cap - 256) / wrec);
         const size_t wlds = ksgw::walk_lds_bytes(pb.d, n, nbuf);
         const long long wchunk = std::max<long long>(
-            1, std::min<long long>(pb.P, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
-        const unsigned wtie_cap =
-            (unsigned)std::min<long long>(tie_cap, std::max<long long>(1024, wchunk * (long long)n / 32));
-        for (long long p0 = 0; p0 < pb.P; p0 += wchunk) {
-            const long long Pc = std::min(wchunk, pb.P - p0);
-            hipEvent_t e0 = g_phase_events.get(evi++), e1 = g_phase_events.get(evi++),
-                       e2 = g_phase_events.get(evi++), e3 = g_phase_events.get(evi++);
-            ksgw::WalkArgs wa;
-            wa.X = X;
-            wa.ld = pb.ld;
-            wa.dstride = pb.dstride;
-            wa.ab = pb.ab;
-            wa.p0 = p0;
-            wa.P = Pc;
-            wa.n = n;
-            wa.k = pb.k;
-            wa.ord = ord;
-            wa.rank = rank;
-            wa.rs = rs;
-            wa.slot_of = slot_of;
-            wa.cnt = cnt;
-            wa.stats = stats;
-            wa.tie_list = tie_list;
-            wa.tie_count = tie_count;
-            wa.tie_cap = wtie_cap;
-            wa.nbuf = nbuf;
-            wa.batch = g_walk_batch;
-            wa.checkpoint = 24;
-            wa.span = span;
-            wa.id_bits = ksgw::heap_id_bits(n);
-            wa.code_shift = ksgw::heap_code_shift(n);
-            const long long grid = std::max<long long>(1, std::min<long long>(Pc, grid_max));
-            rc = fill_u32(tie_count, 0u, 1, stream);
-            if (rc) return rc;
-            if (e0) (void)hipEventRecord(e0, stream);
-            hipLaunchKernelGGL(walk, dim3((unsigned)grid), dim3(ksgw::kWave * wpb), wlds, stream,
-                               wa);
-            NSX_HIP(hipGetLastError(), "walk launch");
-            if (e1) (void)hipEventRecord(e1, stream);
-            // Up to 4 waves per workgroup, each with its LDS row and lists.
-            const size_t tie_wave_bytes = ksgw::tie_wave_lds_bytes(n);
-            const int tie_wpb =
-                (int)std::max<size_t>(1, std::min<size_t>(4, lds_cap / tie_wave_bytes));
-            hipLaunchKernelGGL(tiek, dim3((unsigned)(tie_waves / tie_wpb)),
-                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
-                               stream, wa);
-            NSX_HIP(hipGetLastError(), "tie launch");
-            hipLaunchKernelGGL(tieo, dim3((unsigned)(tie_waves / tie_wpb)),
-                               dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
-                               stream, wa);
-            NSX_HIP(hipGetLastError(), "tie overflow launch");
-            if (e2) (void)hipEventRecord(e2, stream);
-            ksgw::SumArgs sa;
-            sa.cnt = cnt;
-            sa.P = Pc;
-            sa.n = n;
-            sa.k = pb.k;
-            sa.psi = d_psi;
-            sa.inv_k = (float)(1.0 / (float)pb.k);
-            sa.mi_out = out + p0;
-            sa.counts_out = counts_out ? counts_out + p0 * n * 2 : nullptr;
-            const long long sum_blocks = (Pc + 64 * kSumWaves - 1) / (64 * kSumWaves);
-            hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)sum_blocks), dim3(64 * kSumWaves),
-                               0, stream, sa);
-            NSX_HIP(hipGetLastError(), "psi sum launch");
-            if (e3) (void)hipEventRecord(e3, stream);
-            marks.push_back({1, e0, e1});
-            marks.push_back({2, e1, e2});
-            marks.push_back({3, e2, e3});
+            1, std::min<long long>(L.chunk, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
+        const unsigned wtie_cap = (unsigned)std::min<long long>(
+            L.tie_cap, std::max<long long>(1024, wchunk * (long long)n / 32));
+        for (const ListOp &op : L.ops) {
+            if (op.build) {
+                const int first = op.block * L.bn;
+                const int count = std::min(L.bn, L.U - first);
+                const int base = op.half * L.bn;
+                hipEvent_t b0 = pm ? pm->ev() : nullptr, b1 = pm ? pm->ev() : nullptr;
+                if (b0) (void)hipEventRecord(b0, stream);
+                if (L.nb > 1) {
+                    hipLaunchKernelGGL(set_slots_kernel, dim3((unsigned)((count + 255) / 256)),
+                                       dim3(256), 0, stream, L.node_of + first, count, base,
+                                       L.rslot_of);
+                    NSX_HIP(hipGetLastError(), "slot update launch");
+                }
+                ksgw::BuildArgs ba;
+                ba.X = X;
+                ba.ld = pb.ld;
+                ba.dstride = pb.dstride;
+                ba.n = n;
+                ba.rs = rs;
+                ba.node_of = L.node_of + first;
+                const int rows_per_pass = ksgw::kBuildWaves / ksgw::build_row_waves(width);
+                ba.groups = (int)std::max<long long>(
+                    1, std::min<long long>((n + rows_per_pass - 1) / rows_per_pass,
+                                           (8LL * cu_count() + count - 1) / std::max(count, 1)));
+                ba.ord = L.ord + (long long)base * slot_entries;
+                ba.rank = L.rank + (long long)base * slot_entries;
+                ba.abort = abort_flag;
+                hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)count * ba.groups)),
+                                   dim3(ksgw::kWave * ksgw::kBuildWaves),
+                                   ksgw::build_lds_bytes(pb.d, n, rs), stream, ba);
+                NSX_HIP(hipGetLastError(), "list build launch");
+                if (b1) (void)hipEventRecord(b1, stream);
+                if (pm) pm->add(0, b0, b1);
+                continue;
+            }
+            for (long long q0 = 0; q0 < op.pc; q0 += wchunk) {
+                const long long Pc = std::min(wchunk, op.pc - q0);
+                const long long p0 = op.p0 + q0;
+                hipEvent_t e0 = pm ? pm->ev() : nullptr, e1 = pm ? pm->ev() : nullptr,
+                           e2 = pm ? pm->ev() : nullptr, e3 = pm ? pm->ev() : nullptr;
+                ksgw::WalkArgs wa;
+                wa.X = X;
+                wa.ld = pb.ld;
+                wa.dstride = pb.dstride;
+                wa.ab = L.ab;
+                wa.p0 = p0;
+                wa.P = Pc;
+                wa.n = n;
+                wa.k = pb.k;
+                wa.ord = L.ord;
+                wa.rank = L.rank;
+                wa.rs = rs;
+                wa.slot_of = L.nb > 1 ? L.rslot_of : L.slot_of;
+                wa.cnt = L.cnt;
+                wa.stats = stats;
+                wa.tie_list = L.tie_list;
+                wa.tie_count = tie_count;
+                wa.tie_cap = wtie_cap;
+                wa.nbuf = nbuf;
+                wa.batch = g_walk_batch;
+                wa.checkpoint = 24;
+                wa.span = span;
+                wa.id_bits = ksgw::heap_id_bits(n);
+                wa.code_shift = ksgw::heap_code_shift(n);
+                wa.abort = abort_flag;
+                if (!L.tables) wa.rank = nullptr;
+                int rc = fill_u32(tie_count, 0u, 1, stream);
+                if (rc) return rc;
+                if (e0) (void)hipEventRecord(e0, stream);
+                if (pb.k > ksg::kMaxK) {
+                    // One wave per query frame; no tie list (ties are resolved
+                    // in place).
+                    const long long grid = std::max<long long>(
+                        1, std::min<long long>(Pc * n, ws.grid_max * 8));
+                    hipLaunchKernelGGL(ws.walk, dim3((unsigned)grid), dim3(ksgw::kWave),
+                                       ksgw::bigk_lds_bytes(n), stream, wa);
+                    NSX_HIP(hipGetLastError(), "large-k launch");
+                    if (e1) (void)hipEventRecord(e1, stream);
+                    if (e2) (void)hipEventRecord(e2, stream);
+                } else {
+                const long long grid = std::max<long long>(1, std::min<long long>(Pc, ws.grid_max));
+                hipLaunchKernelGGL(ws.walk, dim3((unsigned)grid), dim3(ksgw::kWave * ws.wpb), wlds,
+                                   stream, wa);
+                NSX_HIP(hipGetLastError(), "walk launch");
+                if (e1) (void)hipEventRecord(e1, stream);
+                // Up to 4 waves per workgroup, each with its LDS row and lists.
+                const size_t tie_wave_bytes = ksgw::tie_wave_lds_bytes(n);
+                const int tie_wpb =
+                    (int)std::max<size_t>(1, std::min<size_t>(4, lds_cap / tie_wave_bytes));
+                hipLaunchKernelGGL(ws.tiek, dim3((unsigned)(tie_waves / tie_wpb)),
+                                   dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
+                                   stream, wa);
+                NSX_HIP(hipGetLastError(), "tie launch");
+                hipLaunchKernelGGL(ws.tieo, dim3((unsigned)(tie_waves / tie_wpb)),
+                                   dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
+                                   stream, wa);
+                NSX_HIP(hipGetLastError(), "tie overflow launch");
+                if (e2) (void)hipEventRecord(e2, stream);
+                }
+                ksgw::SumArgs sa;
+                sa.cnt = L.cnt;
+                sa.P = Pc;
+                sa.n = n;
+                sa.k = pb.k;
+                sa.psi = d_psi;
+                sa.inv_k = (float)(1.0 / (float)pb.k);
+                sa.mi_out = out;
+                sa.counts_out = counts_out;
+                sa.perm = L.perm;
+                sa.pbase = p0;
+                sa.abort = abort_flag;
+                const long long sum_blocks = (Pc + 64 * kSumWaves - 1) / (64 * kSumWaves);
+                hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)sum_blocks),
+                                   dim3(64 * kSumWaves), 0, stream, sa);
+                NSX_HIP(hipGetLastError(), "psi sum launch");
+                if (e3) (void)hipEventRecord(e3, stream);
+                if (pm) {
+                    pm->add(1, e0, e1);
+                    pm->add(2, e1, e2);
+                    pm->add(3, e2, e3);
+                }
+            }
         }
     }
+    return kOk;
+}
+
+thread_local int g_last_blocks = 0;
+thread_local long long g_last_slots = 0, g_last_node_builds = 0;
+
+// Runs the whole problem on the list path when it applies (windows of at
+// most ksgw::kMaxFrames frames whose pair records fit LDS, room for at least
+// two nodes' tables, finite coordinates); otherwise returns kOk with
+// *ran == false. Windows run one after another, each with its own tables
+// (the sorted lists depend on the frame set), into mi_out[rep * P + p].
+int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_out,
+              hipStream_t stream, bool *ran) {
+    *ran = false;
+    if (g_method == 1 && pb.k <= ksg::kMaxK) return kOk;
+    const size_t lds_cap = 160 * 1024;
+    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    const bool fits = pb.k > ksg::kMaxK ? ksgw::bigk_lds_bytes(pb.nmax) <= lds_cap
+                                        : rec_bytes + 256 <= lds_cap;
+    const bool eligible = pb.nmax <= ksgw::kMaxFrames && fits &&
+                          (pb.R == 1 || counts_out == nullptr);
+    if (!eligible) {
+        NSX_REQUIRE(pb.k <= ksg::kMaxK,
+                    "ksg: k > 16 needs windows of at most 8192 frames and, with several "
+                    "windows, no per-frame counts");
+        NSX_REQUIRE(g_method != 2,
+                    "ksg: the list path needs windows of at most 8192 frames whose two rows "
+                    "fit in LDS (2*d*n floats <= 160 KiB)");
+        return kOk;
+    }
+    WalkShape ws;
+    int rc = walk_shape(pb, ws);
+    if (rc) return rc;
+    for (int t = 0; t < 4; ++t) g_last_phase_ms[t] = 0;
+    PhaseMarks pm;
+    pm.on = true;
+    hipEvent_t ev_start = g_phase_events.get(0);
+    if (ev_start) (void)hipEventRecord(ev_start, stream);
+
+    ListLayout L;
+    bool ok = false;
+    rc = list_prepare(pb, L, stream, &ok);
+    if (rc || !ok) return rc;
+    hipEvent_t ev_prep = pm.ev();
+    if (ev_prep) (void)hipEventRecord(ev_prep, stream);
+    pm.add(0, ev_start, ev_prep);  // the node map and pair order count as table set-up
+    rc = list_enqueue(pb, L, ws, d_psi, mi_out, counts_out, stream, &pm);
+    if (rc) return rc;
     unsigned long long h_stats[2] = {0, 0};
-    NSX_HIP(hipMemcpyAsync(h_stats, stats, sizeof(h_stats), hipMemcpyDeviceToHost, stream),
+    NSX_HIP(hipMemcpyAsync(h_stats, L.flags + 8, sizeof(h_stats), hipMemcpyDeviceToHost, stream),
             "walk stats readback");
     NSX_HIP(hipStreamSynchronize(stream), "walk sync");
-    for (const Mark &m : marks) {
+    for (const PhaseMarks::Mark &m : pm.marks) {
         float ms = 0;
-        if (m.a && m.b && hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess)
-            g_last_phase_ms[m.phase] += ms;
+        if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) g_last_phase_ms[m.phase] += ms;
     }
     *ran = true;
     g_last_method = 2;
     g_last_entries = (long long)h_stats[1];
     g_last_ties = (long long)h_stats[0];
+    g_last_blocks = L.nb;
+    g_last_slots = L.S;
+    g_last_node_builds = L.node_builds * pb.R;
     return kOk;
 }
 
@@ -738,7 +1065,7 @@ int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t str
     g_last_method = 0;
     g_last_entries = 0;
     if (pb.P == 0) return kOk;
-    if (g_method != 1) {
+    if (g_method != 1 || pb.k > ksg::kMaxK) {
         std::vector<float> psi(pb.nmax + 1);
         psi_table(psi.data(), pb.nmax);
         float *d_psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));
@@ -748,6 +1075,7 @@ int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t str
         bool ran = false;
         rc = run_lists(pb, d_psi, mi_out, counts_out, stream, &ran);
         if (rc || ran) return rc;
+        NSX_REQUIRE(pb.k <= ksg::kMaxK, "ksg: internal: no path for k > 16");
     }
     g_last_method = 1;
     const Config c = config_for(pb);
@@ -820,11 +1148,14 @@ int nsx_ksg_mi(const float *d_X, long long ld, long long dstride, int d, int n,
                void *stream) {
     TraceRange trace("nsx_ksg_mi");
     clear_error();
+    const auto t0 = std::chrono::steady_clock::now();
     const int off = 0;
     Problem pb;
     int rc = make_problem(pb, d_X, ld, dstride, d, &off, &n, 1, d_ab, P, k);
-    if (rc) return rc;
-    return run_eager(pb, d_mi_out, d_counts_out, (hipStream_t)stream);
+    if (!rc) rc = run_eager(pb, d_mi_out, d_counts_out, (hipStream_t)stream);
+    g_last_call_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
 }
 
 int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
@@ -833,10 +1164,13 @@ int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
     TraceRange trace("nsx_ksg_mi_windows");
     clear_error();
     NSX_REQUIRE(h_win_off && h_win_n && R >= 1, "ksg windows: bad window arrays");
+    const auto t0 = std::chrono::steady_clock::now();
     Problem pb;
     int rc = make_problem(pb, d_X, ld, dstride, d, h_win_off, h_win_n, R, d_ab, P, k);
-    if (rc) return rc;
-    return run_eager(pb, d_mi_out, nullptr, (hipStream_t)stream);
+    if (!rc) rc = run_eager(pb, d_mi_out, nullptr, (hipStream_t)stream);
+    g_last_call_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
 }
 
 int nsx_ksg_last_tie_frames(long long *count) {
@@ -869,6 +1203,18 @@ int nsx_ksg_last_phase_ms(double *phase_ms) {
     return kOk;
 }
 
+int nsx_ksg_last_call_ms(double *wall_ms) {
+    if (wall_ms) *wall_ms = g_last_call_ms;
+    return kOk;
+}
+
+int nsx_ksg_last_list_info(int *blocks, long long *slots, long long *node_builds) {
+    if (blocks) *blocks = g_last_method == 2 ? g_last_blocks : 0;
+    if (slots) *slots = g_last_method == 2 ? g_last_slots : 0;
+    if (node_builds) *node_builds = g_last_method == 2 ? g_last_node_builds : 0;
+    return kOk;
+}
+
 int nsx_ksg_set_config(int qpl, int chunk, int wpb, int blocks_per_cu, int symmetric) {
     clear_error();
     std::lock_guard<std::mutex> lock(g_tune_mutex);
@@ -892,6 +1238,7 @@ int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long
     clear_error();
     NSX_REQUIRE(plan && d_mi_out && h_win_off && h_win_n && R >= 1 && P >= 1,
                 "nsx_ksg_plan_create: invalid arguments");
+    NSX_REQUIRE(k <= ksg::kMaxK, "nsx_ksg_plan_create: batched plans support 1 <= k <= 16");
     *plan = nullptr;
     int rc = ensure_device();
     if (rc) return rc;

@@ -82,7 +82,8 @@ struct BuildArgs {
     int groups;         // workgroups per node
     uint16_t *ord;      // [slot][i][rs] frames sorted by (D, j)
     uint16_t *rank;     // [slot][i][rs] upper-bound ranks, indexed by frame
-    unsigned *bad;      // set to 1 when a coordinate is not finite
+    const unsigned *abort;  // nullable: return at once when *abort != 0 (a
+                            // non-finite coordinate was found beforehand)
 };
 
 // Packed counts of a tied query the tie list had no room for: the counts
@@ -108,20 +109,27 @@ struct WalkArgs {
     int nbuf;                   // pair buffers per workgroup (LDS ring)
     int batch;                  // lanes that wait before a refill round
     int checkpoint;             // loop passes between workgroup checkpoints
-    const float *span;          // per table slot: max over d of (max - min) of coordinate d
+    const float *span;          // per node: max over d of (max - min) of coordinate d
     int id_bits;                // frame bits of a heap entry (2^id_bits >= n)
     int code_shift;             // float-bit units per distance code step: 2^code_shift
+    const unsigned *abort;      // nullable: the kernels return at once when *abort != 0
+                                // (a non-finite coordinate; the all-frames kernel runs)
 };
 
-// Frame-order psi sums of the packed counts (one lane per pair).
+// Frame-order psi sums of the packed counts (one lane per pair). Local pair
+// p of the launch is pair pbase + p of the pair list, whose output index is
+// perm[pbase + p] (perm nullable: the index itself).
 struct SumArgs {
     const unsigned *cnt;
     long long P;
     int n, k;
     const float *psi;
     float inv_k;
-    float *mi_out;    // [P]
-    int *counts_out;  // nullable: [P][n][2]
+    float *mi_out;    // [pairs]
+    int *counts_out;  // nullable: [pairs][n][2]
+    const int *perm;
+    long long pbase;
+    const unsigned *abort;  // nullable, as in WalkArgs
 };
 
 using BuildFn = void (*)(BuildArgs);
@@ -191,6 +199,10 @@ WalkFn walk_d3(int k);
 TieFn tie_d1(int k, bool overflow);
 TieFn tie_d2(int k, bool overflow);
 TieFn tie_d3(int k, bool overflow);
+// The one-wave-per-query kernel for k > ksg::kMaxK (any k).
+WalkFn bigk_d1();
+WalkFn bigk_d2();
+WalkFn bigk_d3();
 
 #if defined(__HIPCC__)
 
@@ -428,14 +440,11 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
     const float *x = a.X + (long long)a.node_of[slot] * a.ld;
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = threadIdx.x / kWave;
-    bool bad = false;
+    if (a.abort && *a.abort) return;
     for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
         const int d = t / n, f = t - d * n;
-        const float v = x[d * a.dstride + f];
-        bad |= !__builtin_isfinite(v);
-        lds[t] = v;
+        lds[t] = x[d * a.dstride + f];
     }
-    if (__ballot(bad) && lane == 0) atomicOr(a.bad, 1u);
     __syncthreads();
     uint16_t *st = reinterpret_cast<uint16_t *>(
                        reinterpret_cast<char *>(lds) + ((D * n * 4 + 15) & ~15)) +
@@ -445,10 +454,13 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
         float q[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = lds[d * n + i];
+        // The sort takes any initial placement: frame r * 64 + lane starts in
+        // register r of this lane, so each LDS read covers 64 consecutive
+        // words (no bank conflicts).
         unsigned v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int e = lane * R + r;
+            const int e = r * kWave + lane;
             if (e < n) {
                 float c[D];
 #pragma unroll
@@ -588,12 +600,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
     const int wib = threadIdx.x / kWave;
     const int rg = wib / W, wl = wib - rg * W;  // row in the workgroup, block in the row
     const int ebase = wl * 2048;
-    bool bad = false;
-    for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
-        const int d = t / n, f = t - d * n;
-        bad |= !__builtin_isfinite(x[d * a.dstride + f]);
-    }
-    if (__ballot(bad) && lane == 0) atomicOr(a.bad, 1u);
+    if (a.abort && *a.abort) return;
     unsigned long long *xr = xch + rg * (W * 2048);
     uint16_t *st = reinterpret_cast<uint16_t *>(xr);  // rank staging after the sort
     for (int base = g * kRows; base < n; base += a.groups * kRows) {
@@ -602,10 +609,12 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         float q[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = row ? x[d * a.dstride + i] : 0.0f;
+        // Any initial placement sorts the same: frame ebase + r * 64 + lane
+        // starts in register r of this lane, so each load is coalesced.
         unsigned long long v[32];
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
-            const int e = ebase + lane * 32 + r;
+            const int e = ebase + r * kWave + lane;
             if (row && e < n) {
                 float c[D];
 #pragma unroll
@@ -784,11 +793,10 @@ __device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *
         rec[2 * D * f + D + d] = xb[d * a.dstride + f];
     }
     if (threadIdx.x == 0) {
-        const int sa = a.slot_of[na], sb = a.slot_of[nb];
-        meta[3 * b] = sa;
-        meta[3 * b + 1] = sb;
+        meta[3 * b] = a.slot_of[na];
+        meta[3 * b + 1] = a.slot_of[nb];
         const long long window = 1LL << (32 - a.id_bits + a.code_shift);
-        const long long top = (long long)__float_as_uint(fmaxf(a.span[sa], a.span[sb]));
+        const long long top = (long long)__float_as_uint(fmaxf(a.span[na], a.span[nb]));
         meta[3 * b + 2] = (int)(unsigned)(top - (window - 1) > 0 ? top - (window - 1) : 0);
     }
 }
@@ -804,6 +812,7 @@ __device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *
 template <int D, int K>
 __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
+    if (a.abort && *a.abort) return;
     const int n = a.n;
     const int nbuf = a.nbuf;
     const int rec_floats = 2 * D * n;
@@ -1339,6 +1348,7 @@ __device__ void resolve_tie(const WalkArgs &a, long long pl, int i, char *wave_l
 template <int D, int K>
 __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) char tie_lds[];
+    if (a.abort && *a.abort) return;
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
@@ -1355,7 +1365,7 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
 // to do unless the list overflowed. Each wave scans 64-entry spans of cnt.
 template <int D, int K>
 __global__ __launch_bounds__(256) void tie_overflow_kernel(WalkArgs a) {
-    if (*a.tie_count <= a.tie_cap) return;
+    if ((a.abort && *a.abort) || *a.tie_count <= a.tie_cap) return;
     extern __shared__ __attribute__((aligned(16))) char tie_lds[];
     const int lane = threadIdx.x & (kWave - 1);
     const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -1373,6 +1383,159 @@ __global__ __launch_bounds__(256) void tie_overflow_kernel(WalkArgs a) {
             marked &= marked - 1;
         }
     }
+}
+
+// ---- k above ksg::kMaxK: one wave per query frame ------------------------------
+//
+// The register heaps of the walk and of ksg_kernel.h hold at most kMaxK + 1
+// entries; the reference selects any k (k + 1 argmin sweeps,
+// mutual_information_gpu.cu:62-687). Here a wave takes one query (pair pl,
+// frame i) at a time: it writes the joint distances z_j of the row to LDS,
+// finds t = the float bits of the k-th smallest z_j over the other frames
+// (j != i) by bisection on the bits (each step one pass of compares and a
+// wave sum: z >= 0, so its bits order like its value), and counts the
+// others at or below t. When that count is k, the k nearest frames are
+// exactly those at or below t; when it exceeds k the k-th and (k+1)-th
+// distances tie, and the row's positions 0..k are selected in libstdc++
+// std::sort order (sort_prefix_wave, mutual_information_cpu.cpp:48-77).
+// eps_x / eps_y are the largest marginal distances of the selected frames.
+// The counts come from the rank tables when the tables exist (a.rank, the
+// list path), else from one more pass over the row (the all-frames path).
+// Coordinates are read from global memory; consecutive waves take
+// consecutive frames of one pair, so its rows stay in cache.
+NSX_HD inline size_t bigk_lds_bytes(int n) {
+    return (((size_t)n * 4 + 15) & ~(size_t)15) + tie_wave_lds_bytes(n);
+}
+
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += (unsigned)__shfl_xor((int)v, off);
+    return v;
+}
+
+template <int D>
+__global__ __launch_bounds__(kWave) void bigk_kernel(WalkArgs a) {
+    extern __shared__ __attribute__((aligned(16))) char bigk_lds[];
+    if (a.abort && *a.abort) return;
+    const int lane = threadIdx.x;
+    const int n = a.n, k = a.k;
+    float *zk = reinterpret_cast<float *>(bigk_lds);
+    char *wl = bigk_lds + (((size_t)n * 4 + 15) & ~(size_t)15);
+    KeyIdx *row = reinterpret_cast<KeyIdx *>(wl);
+    uint16_t *lasc = reinterpret_cast<uint16_t *>(wl + (size_t)n * sizeof(KeyIdx));
+    uint16_t *rasc = lasc + n;
+    int *stk = reinterpret_cast<int *>(rasc + n);
+    const long long total = a.P * n;
+    unsigned long long ties = 0;
+    for (long long w = blockIdx.x; w < total; w += gridDim.x) {
+        const long long pl = w / n;
+        const int i = (int)(w - pl * n);
+        const long long p = a.p0 + pl;
+        const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
+        const float *xa = a.X + (long long)na * a.ld;
+        const float *xb = a.X + (long long)nb * a.ld;
+        float qa[D], qb[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            qa[d] = xa[d * a.dstride + i];
+            qb[d] = xb[d * a.dstride + i];
+        }
+        for (int j = lane; j < n; j += kWave) {
+            float va[D], vb[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                va[d] = xa[d * a.dstride + j];
+                vb[d] = xb[d * a.dstride + j];
+            }
+            zk[j] = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
+        }
+        ksg::wave_sync();
+        // Smallest t with #{j != i : bits(z_j) <= t} >= k (k <= n - 1 others).
+        unsigned lo = 0u, hi = 0xffffffffu;
+        while (lo < hi) {
+            const unsigned mid = lo + ((hi - lo) >> 1);
+            unsigned c = 0;
+            for (int j = lane; j < n; j += kWave)
+                c += (j != i && __float_as_uint(zk[j]) <= mid) ? 1u : 0u;
+            c = wave_sum_u32(c);
+            if (c >= (unsigned)k) hi = mid;
+            else lo = mid + 1u;
+        }
+        const unsigned t = lo;
+        unsigned c_le = 0;
+        for (int j = lane; j < n; j += kWave)
+            c_le += (j != i && __float_as_uint(zk[j]) <= t) ? 1u : 0u;
+        c_le = wave_sum_u32(c_le);
+        const bool tie = c_le > (unsigned)k;
+        ties += tie ? 1u : 0u;
+        float ex = -1.0f, ey = -1.0f;
+        int jx = 0, jy = 0;
+        auto take = [&](int j) {
+            float va[D], vb[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                va[d] = xa[d * a.dstride + j];
+                vb[d] = xb[d * a.dstride + j];
+            }
+            const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+            jx = da > ex ? j : jx;
+            ex = fmaxf(ex, da);
+            jy = db > ey ? j : jy;
+            ey = fmaxf(ey, db);
+        };
+        if (!tie) {
+            for (int j = lane; j < n; j += kWave)
+                if (j != i && __float_as_uint(zk[j]) <= t) take(j);
+        } else {
+            // The row as the reference sorts it (self included).
+            for (int j = lane; j < n; j += kWave) {
+                KeyIdx r;
+                r.key = zk[j];
+                r.idx = j;
+                row[j] = r;
+            }
+            ksg::wave_sync();
+            sort_prefix_wave(row, n, k + 1, lasc, rasc, stk, lane);
+            for (int m = 1 + lane; m <= k; m += kWave) take(row[m].idx);
+        }
+        // Largest marginal distances over the wave (any frame of the largest
+        // distance gives the same rank).
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const float oex = __shfl_xor(ex, off), oey = __shfl_xor(ey, off);
+            const int ojx = __shfl_xor(jx, off), ojy = __shfl_xor(jy, off);
+            jx = oex > ex ? ojx : jx;
+            ex = fmaxf(ex, oex);
+            jy = oey > ey ? ojy : jy;
+            ey = fmaxf(ey, oey);
+        }
+        unsigned nx, ny;
+        if (a.rank) {
+            const long long offA = table_row(a.slot_of[na], i, n, a.rs);
+            const long long offB = table_row(a.slot_of[nb], i, n, a.rs);
+            nx = (unsigned)a.rank[table_at(offA, jx)] - 1u;
+            ny = (unsigned)a.rank[table_at(offB, jy)] - 1u;
+        } else {
+            // <= eps over every frame, self included, minus one
+            // (mutual_information_cpu.cpp:94-109).
+            unsigned cx = 0, cy = 0;
+            for (int j = lane; j < n; j += kWave) {
+                float va[D], vb[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    va[d] = xa[d * a.dstride + j];
+                    vb[d] = xb[d * a.dstride + j];
+                }
+                cx += marg<D>(qa, va) <= ex ? 1u : 0u;
+                cy += marg<D>(qb, vb) <= ey ? 1u : 0u;
+            }
+            nx = wave_sum_u32(cx) - 1u;
+            ny = wave_sum_u32(cy) - 1u;
+        }
+        if (lane == 0) a.cnt[pl * n + i] = nx | (ny << 16);
+        ksg::wave_sync();
+    }
+    if (lane == 0) atomicAdd(a.stats, ties);
 }
 
 template <int D, int K>

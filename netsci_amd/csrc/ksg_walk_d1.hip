@@ -8,6 +8,7 @@ namespace ksgw {
 BuildFn build_d1(int width) { return build_lookup<1>(width); }
 WalkFn walk_d1(int k) { return walk_lookup<1>(k); }
 TieFn tie_d1(int k, bool overflow) { return tie_lookup<1>(k, overflow); }
+WalkFn bigk_d1() { return bigk_kernel<1>; }
 
 }  // namespace ksgw
 }  // namespace nsx

@@ -8,6 +8,7 @@ namespace ksgw {
 BuildFn build_d3(int width) { return build_lookup<3>(width); }
 WalkFn walk_d3(int k) { return walk_lookup<3>(k); }
 TieFn tie_d3(int k, bool overflow) { return tie_lookup<3>(k, overflow); }
+WalkFn bigk_d3() { return bigk_kernel<3>; }
 
 }  // namespace ksgw
 }  // namespace nsx

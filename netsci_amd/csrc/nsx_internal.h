@@ -61,6 +61,19 @@ int cu_count();
 // Returns nullptr (and sets the error) on allocation failure.
 void *workspace(int slot, size_t bytes);
 
+// The sorted-list tables of the KSG list path: one grow-only device buffer
+// counted in node slots of slot_bytes each. Returns in *ptr / *slots the
+// buffer and the slots it holds, at most want_slots. The buffer grows (the
+// only time device memory is queried or allocated) when it holds fewer than
+// want_slots and the budget allows more; *slots < min_slots means no room.
+// The budget is set_table_budget()'s, or 85% of the device memory free
+// (counting the current buffer) less 1 GiB.
+int table_space(size_t slot_bytes, long long want_slots, long long min_slots, void **ptr,
+                long long *slots);
+void set_table_budget(size_t bytes);
+// Frees every workspace and the table buffer (the device must be idle).
+void release_workspaces();
+
 enum WorkspaceSlot : int {
     kWsPsi = 0,
     kWsCounter = 1,
@@ -79,6 +92,8 @@ enum WorkspaceSlot : int {
     kWsListMap = 14,
     kWsListCnt = 15,
     kWsListStats = 16,
+    kWsListPerm = 17,
+    kWsListSpan = 18,
     kWsCount = 20,
 };
 

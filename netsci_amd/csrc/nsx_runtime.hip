@@ -65,6 +65,69 @@ void *workspace(int slot, size_t bytes) {
     return p;
 }
 
+static void *g_tab_ptr = nullptr;
+static size_t g_tab_bytes = 0;
+static size_t g_tab_budget = 0;  // 0: the default budget
+
+void set_table_budget(size_t bytes) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    g_tab_budget = bytes;
+}
+
+int table_space(size_t slot_bytes, long long want_slots, long long min_slots, void **ptr,
+                long long *slots) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    *ptr = nullptr;
+    *slots = 0;
+    if (slot_bytes == 0 || want_slots <= 0) return kOk;
+    long long have = (long long)(g_tab_bytes / slot_bytes);
+    if (have < want_slots) {
+        size_t free_b = 0, total_b = 0;
+        NSX_HIP(hipMemGetInfo(&free_b, &total_b), "memory info");
+        const double avail = (double)free_b + (double)g_tab_bytes;
+        double budget = 0.85 * avail - (double)(1ull << 30);
+        if (g_tab_budget) budget = std::min(budget, (double)g_tab_budget);
+        long long target = budget > 0 ? (long long)(budget / (double)slot_bytes) : 0;
+        target = std::min(target, want_slots);
+        if (target > have && target >= min_slots) {
+            // The current buffer may still be read by queued work.
+            NSX_HIP(hipDeviceSynchronize(), "table buffer sync");
+            if (g_tab_ptr) (void)hipFree(g_tab_ptr);
+            g_tab_ptr = nullptr;
+            g_tab_bytes = 0;
+            while (target >= min_slots && target > 0) {
+                void *p = nullptr;
+                const size_t bytes = (size_t)target * slot_bytes + 256;
+                if (hipMalloc(&p, bytes) == hipSuccess) {
+                    g_tab_ptr = p;
+                    g_tab_bytes = bytes;
+                    break;
+                }
+                (void)hipGetLastError();
+                target = target * 3 / 4;
+            }
+            have = (long long)(g_tab_bytes / slot_bytes);
+        }
+    }
+    // A budget below the buffer held applies at once.
+    if (g_tab_budget) have = std::min(have, (long long)(g_tab_budget / slot_bytes));
+    *ptr = g_tab_ptr;
+    *slots = std::min(have, want_slots);
+    return kOk;
+}
+
+void release_workspaces() {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    for (int s = 0; s < kWsCount; ++s) {
+        if (g_ws_ptr[s]) (void)hipFree(g_ws_ptr[s]);
+        g_ws_ptr[s] = nullptr;
+        g_ws_bytes[s] = 0;
+    }
+    if (g_tab_ptr) (void)hipFree(g_tab_ptr);
+    g_tab_ptr = nullptr;
+    g_tab_bytes = 0;
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, uint32_t v, size_t count) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -365,6 +428,21 @@ int nsx_probe_valu(int kind, int iters, double *ms, long long *lanes) {
     (void)hipEventDestroy(e1);
     *ms = t;
     *lanes = (long long)blocks * 256;
+    return kOk;
+}
+
+int nsx_release_workspace(void) {
+    clear_error();
+    if (g_device < 0) return kOk;
+    NSX_HIP(hipDeviceSynchronize(), "release sync");
+    release_workspaces();
+    return kOk;
+}
+
+int nsx_ksg_set_table_budget(long long bytes) {
+    clear_error();
+    NSX_REQUIRE(bytes >= 0, "nsx_ksg_set_table_budget: bytes must be >= 0");
+    set_table_budget((size_t)bytes);
     return kOk;
 }
 

@@ -186,23 +186,33 @@ void orc_hedetniemi(const float *A, int N, int L, float tol, float *H_out,
     std::vector<float> Hi;
     bool want_paths = paths != nullptr;
     if (want_paths || Hi_out) Hi.resize((size_t)L * NN);
+    // HT[i][k] = H[k][i]: the same candidates in the same ascending k order,
+    // read from a row; rows i are independent, so threads change nothing.
+    std::vector<float> HT(NN);
     for (int p = 0; p < L; p++) {
         if (!Hi.empty()) std::copy(H.begin(), H.end(), Hi.begin() + (size_t)p * NN);
-        for (int i = 0; i < N; i++)
+        for (int k = 0; k < N; k++)
+            for (int i = 0; i < N; i++) HT[(long long)i * N + k] = H[(long long)k * N + i];
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < N; i++) {
+            const float *hti = HT.data() + (long long)i * N;
             for (int j = 0; j < N; j++) {
+                const float *aj = A + (long long)j * N;
                 float c = INFINITY;
                 for (int k = 0; k < N; k++) {
-                    float s = A[(long long)j * N + k] + H[(long long)k * N + i];
+                    float s = aj[k] + hti[k];
                     if (s < c) c = s;
                 }
                 Hn[(long long)i * N + j] = c;
             }
+        }
         H.swap(Hn);
     }
     std::copy(H.begin(), H.end(), H_out);
     if (Hi_out) std::copy(Hi.begin(), Hi.end(), Hi_out);
     if (!want_paths) return;
     std::fill(paths, paths + (size_t)N * L * N, -1);
+#pragma omp parallel for schedule(dynamic, 4)
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
             int *pp = paths + (long long)i * N * L + (long long)j * L;

@@ -1,11 +1,13 @@
 """Checkpointed GC on the GPU platform (GPU required): the file protocol of
-generalized_correlation.cpp:8-101 and the values equal the CPU platform's
-run of the same call."""
+generalized_correlation.cpp:8-101, the values equal to the oracle's pair
+loop (the reference CPU path restated) and to the CPU platform's run of the
+same call."""
 import os
 
 import numpy as np
 import pytest
 
+import oracle
 from netsci_amd import cuarray, netcalc, synth
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +41,10 @@ def test_gpu_checkpoints_equal_cpu(tmp_path):
         assert rc == plat
         out[sub] = (R.toNumpy1D(), sorted(os.listdir(tmp_path / sub)),
                     np.load(str(tmp_path / sub / "ck_20.npy")))
+    want = oracle.pairs(X, ab, k, n, d, gc=True)
+    np.testing.assert_array_equal(out["gpu"][0], want)
     np.testing.assert_array_equal(out["gpu"][0], out["cpu"][0])
+    # The last checkpoint (after pair 20) holds every value.
+    np.testing.assert_array_equal(out["gpu"][2].ravel(), want)
     assert out["gpu"][1] == out["cpu"][1] == ["ck_20.npy", "ck_ab.npy"]
     np.testing.assert_array_equal(out["gpu"][2], out["cpu"][2])

@@ -1,0 +1,308 @@
+"""Every benchmark configuration at its stated size on the GPU (GPU required).
+
+* C0: 64 Gaussian nodes, d=1, n=1000, k=4, all 2,016 pairs, through
+  netcalc.mutualInformation on both platforms: both bit-exact against the
+  oracle's pair loop, plus the Gaussian closed form -1/2 ln(1 - rho^2) as a
+  sanity check (KSG bias allowed).
+* C1: N=1000 C-alpha nodes, d=3, n=2000, k=6, all 499,500 pairs on the list
+  path: MI(a, b) == MI(b, a) bitwise over every pair (a second call with the
+  pairs reversed), and 128 seeded pairs bit-exact against the oracle.
+* C2: Hedetniemi on the -log(GC) contact graph (mean distance < 7.5 A) at
+  N=1000 (H, every Hi[p] and the paths bit-exact against the oracle's
+  Jacobi restatement, L = hop diameter + 1) and at N=4000 (H >= the
+  converged distances, which equal scipy's Dijkstra within float tolerance;
+  symmetry to rounding; 1,000 seeded decoded paths are edge walks whose
+  weights sum to H).
+* C3: 512 nodes, 64 windows n_r ~ U[500, 4000] of a 4000-frame trajectory:
+  the graph-captured all-frames plan, the windowed list-path call and one
+  list-path call per window agree bitwise on all 64 x 130,816 items, and 16
+  seeded (window, pair) items equal the oracle.
+* C4: N=4000, d=3, n=5000, k=6, all 7,998,000 pairs: the plain pass, then
+  the screened pass (screener trained on 8 other trajectories): recall of
+  GC > 0.3 >= 0.995, every computed value bit-identical to the plain pass,
+  and 8 seeded pairs of the plain pass equal the oracle.
+
+The oracle (oracle/ksg_oracle.cpp, the reference CPU path restated) runs on
+seeded samples in a thread pool.
+"""
+import concurrent.futures as cf
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from netsci_amd import abi, cuarray, netcalc, synth
+
+pytestmark = pytest.mark.gpu
+
+_THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _oracle_mi(X, ab, k, n, d):
+    """oracle.pairs (MI) of every row of ab, one pair per task."""
+    ab = np.ascontiguousarray(ab, np.int32).reshape(-1, 2)
+    with cf.ThreadPoolExecutor(_THREADS) as ex:
+        out = list(ex.map(lambda r: oracle.pairs(X, r.reshape(1, 2), k, n, d, gc=False)[0], ab))
+    return np.array(out, np.float32)
+
+
+def _device_mi(dX, ld, dstride, d, n, ab, k):
+    dab = abi.DeviceArray.from_numpy(np.ascontiguousarray(ab, np.int32))
+    dmi = abi.DeviceArray((len(ab),), np.float32)
+    abi.ksg_mi(dX, ld, dstride, d, n, dab, len(ab), k, dmi)
+    return dmi.numpy()
+
+
+@pytest.fixture(autouse=True)
+def _auto():
+    abi.set_method(abi.METHOD_AUTO)
+    abi.set_table_budget(0)
+    yield
+    abi.set_method(abi.METHOD_AUTO)
+
+
+# ---- C0 ------------------------------------------------------------------------
+
+
+def test_c0_gaussian_both_platforms():
+    M, n, d, k = 64, 1000, 1, 4
+    _flush("C0: 64 Gaussian nodes, 2016 pairs, both platforms")
+    X, rho = synth.gaussian_nodes(M=M, n=n, seed=0)
+    ab = synth.all_pairs(M)
+    Xc, abc = cuarray.FloatCuArray(), cuarray.IntCuArray()
+    Xc.fromNumpy2D(X)
+    abc.fromNumpy2D(ab)
+    got = {}
+    for plat in (netcalc.GPU_PLATFORM, netcalc.CPU_PLATFORM):
+        I = cuarray.FloatCuArray()
+        assert netcalc.mutualInformation(X=Xc, I=I, ab=abc, k=k, n=n, xd=2, d=d,
+                                         platform=plat) == plat
+        got[plat] = I.toNumpy1D()
+    want = _oracle_mi(X, ab, k, n, d)
+    np.testing.assert_array_equal(got[netcalc.GPU_PLATFORM], want)
+    np.testing.assert_array_equal(got[netcalc.CPU_PLATFORM], want)
+    r = rho[ab[:, 0], ab[:, 1]]
+    analytic = -0.5 * np.log(1.0 - r ** 2)
+    err = want.astype(np.float64) - analytic
+    # KSG's estimation noise at n = 1000 (on this input: mean |err| 0.019,
+    # max 0.082, bias -0.0006, correlation 0.965).
+    assert np.corrcoef(want, analytic)[0, 1] > 0.95
+    assert np.abs(err).mean() < 0.03 and np.abs(err).max() < 0.15 and abs(err.mean()) < 0.01
+
+
+# ---- C1 and C2 (N = 1000) --------------------------------------------------------
+
+_C1 = {}
+
+
+def _c1():
+    """C1's inputs and its all-pairs MI on the GPU (computed once)."""
+    if not _C1:
+        N, n, d, k = 1000, 2000, 3, 6
+        X, mean = synth.calpha_trajectory(N=N, n=n, seed=1)
+        ab = synth.all_pairs(N)
+        dX = abi.DeviceArray.from_numpy(X)
+        mi = _device_mi(dX, X.shape[1], n, d, n, ab, k)
+        _C1.update(N=N, n=n, d=d, k=k, X=X, mean=mean, ab=ab, dX=dX, mi=mi,
+                   method=abi.last_method()[0], info=abi.last_list_info())
+    return _C1
+
+
+def test_c1_all_pairs_symmetry_and_oracle_sample():
+    _flush("C1: 499,500 pairs, N=1000 n=2000")
+    c = _c1()
+    assert c["method"] == 2 and c["info"][0] == 1  # the list path, every table resident
+    assert len(c["mi"]) == 499500 and np.isfinite(c["mi"]).all()
+    _flush("C1: reversed pairs")
+    rev = _device_mi(c["dX"], c["X"].shape[1], c["n"], c["d"], c["n"], c["ab"][:, ::-1],
+                     c["k"])
+    np.testing.assert_array_equal(rev, c["mi"])
+    _flush("C1: 128 pairs against the oracle")
+    rng = np.random.default_rng(2024)
+    idx = rng.choice(len(c["ab"]), size=128, replace=False)
+    want = _oracle_mi(c["X"], c["ab"][idx], c["k"], c["n"], c["d"])
+    np.testing.assert_array_equal(c["mi"][idx], want)
+
+
+def test_c2_n1000_bitexact():
+    _flush("C2: Hedetniemi N=1000 on C1's contact graph")
+    c = _c1()
+    N, d = c["N"], c["d"]
+    pairs = synth.contact_pairs(c["mean"])
+    pidx = (pairs[:, 0].astype(np.int64) * (2 * N - pairs[:, 0] - 1)) // 2 + \
+        (pairs[:, 1] - pairs[:, 0] - 1)
+    np.testing.assert_array_equal(c["ab"][pidx], pairs)
+    gc = abi.gc_from_mi(c["mi"][pidx], d)
+    A = synth.contact_adjacency(gc, pairs, N)
+    L = synth.hop_diameter(A) + 1
+    tol = 1e-4
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    dP = abi.DeviceArray((N * L * N,), np.int32)
+    abi.hedetniemi(dA, N, L, tol, dH, dHi, dP)
+    H, Hi, paths = dH.numpy(), dHi.numpy(), dP.numpy()
+    _flush(f"C2: oracle N=1000 L={L}")
+    Ho, Hio, Po = oracle.hedetniemi(A, L, tol, want_hi=True)
+    np.testing.assert_array_equal(H, Ho)
+    np.testing.assert_array_equal(Hi, Hio)
+    np.testing.assert_array_equal(paths, Po)
+    dL = abi.DeviceArray((N, N), np.float32)
+    abi.hedetniemi_lengths(dA, N, L, dL)
+    np.testing.assert_array_equal(dL.numpy(), Ho)
+
+
+# ---- C2 (N = 4000) ---------------------------------------------------------------
+
+
+def _decode(row, i, j):
+    """hedetniemi.cpp:71-108 on one paths[i][j][:] row."""
+    path = [i]
+    for v in row:
+        if v not in path:
+            path.append(int(v))
+    if j not in path:
+        path.append(j)
+    return path
+
+
+def test_c2_n4000_properties():
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    N, n, d, k = 4000, 5000, 3, 6
+    _flush("C2: contact-pair GC at N=4000 n=5000")
+    X, mean = synth.calpha_trajectory(N=N, n=n, seed=9001)
+    pairs = synth.contact_pairs(mean)
+    dX = abi.DeviceArray.from_numpy(X)
+    gc = abi.gc_from_mi(_device_mi(dX, X.shape[1], n, d, n, pairs, k), d)
+    del dX
+    A = synth.contact_adjacency(gc, pairs, N)
+    L = synth.hop_diameter(A) + 1
+    tol = 1e-4
+    _flush(f"C2: Hedetniemi N=4000 L={L}, {len(pairs)} contact pairs")
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    dP = abi.DeviceArray((N * L * N,), np.int32)
+    abi.hedetniemi(dA, N, L, tol, dH, dHi, dP)
+    H = dH.numpy()
+    # Lengths with many more steps converge: they equal Dijkstra (float64)
+    # within float tolerance, and L steps never go below them.
+    dC = abi.DeviceArray((N, N), np.float32)
+    abi.hedetniemi_lengths(dA, N, 8 * L, dC)
+    Hc = dC.numpy()
+    Wm = np.where(np.isfinite(A), A, 0).astype(np.float64)
+    np.fill_diagonal(Wm, 0)
+    sp = shortest_path(csr_matrix(Wm), method="D", directed=True)
+    fin = np.isfinite(sp)
+    np.testing.assert_array_equal(np.isfinite(Hc), fin)
+    np.testing.assert_allclose(Hc[fin], sp[fin], rtol=1e-5, atol=1e-5)
+    assert (H[fin] >= Hc[fin]).all() and np.isfinite(H).sum() <= fin.sum()
+    # H[i][j] and H[j][i] add the same path's float weights in opposite
+    # orders, so they agree to rounding, not bitwise.
+    np.testing.assert_array_equal(np.isfinite(H), np.isfinite(H.T))
+    np.testing.assert_allclose(H[fin], H.T[fin], rtol=1e-6, atol=1e-6)
+    # Decoded paths of 1,000 seeded (i, j): edges of the graph, weights
+    # summing to H[i][j].
+    rng = np.random.default_rng(7)
+    ij = rng.integers(0, N, size=(1000, 2))
+    row = np.empty(L, np.int32)
+    checked = 0
+    for i, j in ij:
+        i, j = int(i), int(j)
+        if i == j or not np.isfinite(H[i, j]):
+            continue
+        off = ((i * L * N) + j * L) * 4
+        src = ctypes.c_void_p(dP.ptr.value + off)
+        abi.check(abi.lib().nsx_memcpy_d2h(row.ctypes.data, src, L * 4), "paths row")
+        path = _decode(row, i, j)
+        assert path[0] == i and path[-1] == j
+        w = [A[path[t], path[t + 1]] for t in range(len(path) - 1)]
+        assert np.isfinite(w).all(), (i, j, path)
+        assert abs(float(np.sum(np.float64(w))) - float(H[i, j])) <= 1e-4 * max(1.0, H[i, j]) + \
+            L * tol
+        checked += 1
+    assert checked > 900
+
+
+# ---- C3 ------------------------------------------------------------------------
+
+
+def test_c3_bootstrap_plan_lists_per_window():
+    N, nT, d, k, R = 512, 4000, 3, 6, 64
+    _flush("C3: 512 nodes, 64 windows")
+    X, _ = synth.calpha_trajectory(N=N, n=nT, seed=7)
+    ab = synth.all_pairs(N)
+    P = len(ab)
+    win = synth.bootstrap_windows(R=R, n_total=nT, lo=500, hi=4000, seed=3)
+    dX = abi.DeviceArray.from_numpy(X)
+    dab = abi.DeviceArray.from_numpy(ab)
+    out_plan = abi.DeviceArray((R, P), np.float32)
+    plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, out_plan, autotune=False)
+    plan.launch()
+    abi.sync()
+    got_plan = out_plan.numpy()
+    plan.close()
+    _flush("C3: windows on the list path")
+    out_l = abi.DeviceArray((R, P), np.float32)
+    abi.ksg_mi_windows(dX, X.shape[1], nT, d, win, dab, P, k, out_l)
+    assert abi.last_method()[0] == 2
+    got_lists = out_l.numpy()
+    np.testing.assert_array_equal(got_lists, got_plan)
+    _flush("C3: one call per window")
+    out_w = abi.DeviceArray((R, P), np.float32)
+    for r, (off, ln) in enumerate(win):
+        abi.ksg_mi(dX, X.shape[1], nT, d, int(ln), dab, P, k, out_w, frame_offset=int(off),
+                   out_offset=r * P)
+        assert abi.last_method()[0] == 2
+    np.testing.assert_array_equal(out_w.numpy(), got_plan)
+    _flush("C3: 16 items against the oracle")
+    rng = np.random.default_rng(33)
+    items = [(int(rng.integers(R)), int(rng.integers(P))) for _ in range(16)]
+
+    def one(item):
+        r, p = item
+        off, ln = win[r]
+        Xw = np.concatenate([X[:, di * nT + off: di * nT + off + ln] for di in range(d)], axis=1)
+        return oracle.pairs(Xw, ab[p:p + 1], k, int(ln), d, gc=False)[0]
+
+    with cf.ThreadPoolExecutor(_THREADS) as ex:
+        want = list(ex.map(one, items))
+    for (r, p), w in zip(items, want):
+        assert got_plan[r, p] == np.float32(w)
+
+
+# ---- C4 ------------------------------------------------------------------------
+
+
+def test_c4_screened_vs_plain():
+    from netsci_amd import screen
+    N, n, d, k = 4000, 5000, 3, 6
+    _flush("C4: training the screener (8 trajectories)")
+    feat, y = screen.training_set(trajectories=8, N=1000, n=n, pairs_per=60000, k=k, seed=100)
+    model, _ = screen.train_screener(feat, y, recall=0.999, epochs=40, seed=0)
+    X, _ = synth.calpha_trajectory(N=N, n=n, seed=9001)
+    ab = synth.all_pairs(N)
+    dX = abi.DeviceArray.from_numpy(X)
+    _flush("C4: plain pass, 7,998,000 pairs")
+    mi = _device_mi(dX, X.shape[1], n, d, n, ab, k)
+    assert abi.last_method()[0] == 2
+    plain = abi.gc_from_mi(mi, d)
+    _flush("C4: screened pass")
+    res = screen.screened_gc(X, N, n, k, model, ab=ab, audit_rate=0.01, seed=1, dX=dX)
+    comp = res["computed"]
+    pos = plain > screen.GC_THRESHOLD
+    recall = float(np.sum(pos & comp) / max(int(pos.sum()), 1))
+    assert recall >= 0.995, recall
+    assert comp.mean() < 0.2
+    np.testing.assert_array_equal(res["gc"][comp], plain[comp])
+    _flush("C4: 8 pairs against the oracle")
+    rng = np.random.default_rng(4)
+    idx = rng.choice(len(ab), size=8, replace=False)
+    np.testing.assert_array_equal(mi[idx], _oracle_mi(X, ab[idx], k, n, d))

@@ -226,8 +226,8 @@ def test_invalid_arguments_raise():
     dX = abi.DeviceArray.from_numpy(X)
     dab = abi.DeviceArray.from_numpy(np.array([[0, 1]], np.int32))
     dmi = abi.DeviceArray((1,), np.float32)
-    with pytest.raises(RuntimeError, match="k"):
-        abi.ksg_mi(dX, 100, 100, 1, 100, dab, 1, 17, dmi)
+    with pytest.raises(RuntimeError, match="more than k frames"):
+        abi.ksg_mi(dX, 100, 100, 1, 100, dab, 1, 100, dmi)
     with pytest.raises(RuntimeError, match="d = 1, 2 or 3"):
         abi.ksg_mi(dX, 100, 25, 4, 25, dab, 1, 4, dmi)
 

@@ -82,6 +82,25 @@ def test_invalid_platform_and_arguments_raise():
                                        platform=1)
     with pytest.raises(ValueError):
         netcalc.generalizedCorrelation(X=X, R=R, ab=ab, k=4, n=100, xd=2, d=2, platform=1)
+    # xd: the reference's GPU kernels exist only for two variables
+    # (mutual_information_gpu.cu:1015-1030); the check runs before any device
+    # call. Its CPU path ignores xd, and so does the CPU platform.
+    for gpu_call in (
+            lambda: netcalc.generalizedCorrelation(X=X, R=R, ab=ab, k=4, n=100, xd=3, d=1,
+                                                   platform=netcalc.GPU_PLATFORM),
+            lambda: netcalc.mutualInformation(X=X, I=R, ab=ab, k=4, n=100, xd=1, d=1,
+                                              platform=netcalc.GPU_PLATFORM),
+            lambda: netcalc.mutualInformationGpu(Xa=_fa(np.zeros(100)), Xb=_fa(np.zeros(100)),
+                                                 k=4, n=100, xd=3, d=1)):
+        with pytest.raises(ValueError, match="xd must be 2"):
+            gpu_call()
+    Xr = _fa(np.random.default_rng(1).standard_normal((2, 100)))
+    vals = []
+    for xd in (2, 3):
+        netcalc.mutualInformation(X=Xr, I=R, ab=ab, k=4, n=100, xd=xd, d=1,
+                                  platform=netcalc.CPU_PLATFORM)
+        vals.append(R.toNumpy1D()[0])
+    assert vals[0] == vals[1]
     A, H, P = _fa(np.zeros((3, 3))), cuarray.FloatCuArray(), cuarray.IntCuArray()
     with pytest.raises(ValueError, match="Invalid platform"):
         netcalc.hedetniemiAllShortestPaths(A=A, H=H, paths=P, maxPathLength=2, tolerance=0.1,
