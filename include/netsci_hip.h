@@ -96,6 +96,21 @@ int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride,
                        const int *d_ab, long long P, int k, float *d_mi_out,
                        void *stream);
 
+/* Allocation-free variant of nsx_ksg_mi (d_counts_out = NULL). No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same results as nsx_ksg_mi, computed by
+ * the all-frames kernel, whose scratch (psi table, tie counter, per-wave tie
+ * rows) lives in a 64 MiB static device array that each GPU holds until the
+ * process ends; it calls no hipMalloc / hipFree. n <= 262144 frames (larger
+ * n is rejected). The launch is capped to the waves whose tie rows fit
+ * (64 MiB / (8 n) waves). Stream-ordered: it enqueues and returns without
+ * synchronising, and does not update nsx_ksg_last_tie_frames. Every call on
+ * a GPU shares that GPU's array, so calls that overlap on one GPU race. */
+int nsx_ksg_mi_noalloc(const float *d_X, long long ld, long long dstride, int d,
+                       int n, const int *d_ab, long long P, int k, float *d_mi_out,
+                       void *stream);
+
 /* Frames whose k-th / (k+1)-th joint distances tied in the last KSG launch
  * on this thread (resolved on the device in libstdc++ std::sort order). */
 int nsx_ksg_last_tie_frames(long long *count);

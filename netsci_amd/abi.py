@@ -37,6 +37,7 @@ SIGNATURES = {
     "nsx_event_elapsed_ms": [_VP, _VP, _FP],
     "nsx_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP, _VP],
     "nsx_ksg_mi_windows": [_VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I, _VP, _VP],
+    "nsx_ksg_mi_noalloc": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP],
     "nsx_ksg_last_tie_frames": [_LLP],
     "nsx_ksg_set_config": [_I, _I, _I, _I, _I],
     "nsx_ksg_set_method": [_I, _I, _I],

@@ -1108,6 +1108,96 @@ int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t str
     return kOk;
 }
 
+// ---- allocation-free variant of nsx_ksg_mi -------------------------------------
+//
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops, harnesses that forbid hipMalloc): the all-frames
+// kernel with every buffer it needs besides X, ab and the output in one
+// static device array, `noalloc_scratch`, of kNoAllocBytes (each GPU holds
+// its own copy for the life of the process). Layout: [0, 64) the tie
+// counter (two u64), [64, 76) the window record (offset 0, n, replicate 0),
+// [256, ...) the psi table, then per wave one n-record tie row. The launch
+// is capped to the waves whose rows fit. Every call on a GPU shares that
+// GPU's array: calls that overlap on one GPU race.
+
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxFrames = 1 << 18;  // psi (1 MiB) and 4 waves' rows (8 MiB each) fit
+
+__device__ __align__(256) unsigned char noalloc_scratch[kNoAllocBytes];
+
+// Device address of noalloc_scratch on each device (the array's name is a
+// host-side stand-in; kernels take this address). Filled once per device,
+// with that device current.
+std::mutex g_noalloc_mutex;
+void *g_noalloc_ptr[64] = {};
+
+int noalloc_base(void **base) {
+    int dev = 0;
+    NSX_HIP(hipGetDevice(&dev), "hipGetDevice");
+    NSX_REQUIRE(dev >= 0 && dev < 64, "nsx_ksg_mi_noalloc: device index above 63");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        NSX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)),
+                "hipGetSymbolAddress(noalloc_scratch)");
+        g_noalloc_ptr[dev] = p;
+    }
+    *base = g_noalloc_ptr[dev];
+    return kOk;
+}
+
+// psi.cpp:6-19 on the device: the terms (float)(1.0 / (double)(float)i) in
+// parallel, then one lane adds them in order from psi[1] =
+// (float)-0.57721566490153 (the same float sums as psi_table on the host).
+__global__ __launch_bounds__(256) void psi_table_kernel(float *psi, int n) {
+    for (int i = (int)threadIdx.x + 1; i < n; i += (int)blockDim.x)
+        psi[i + 1] = (float)(1.0 / (double)(float)i);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        psi[0] = 0.0f;
+        if (n >= 1) {
+            float acc = (float)-0.57721566490153;
+            psi[1] = acc;
+            for (int i = 1; i < n; ++i) {
+                acc = acc + psi[i + 1];
+                psi[i + 1] = acc;
+            }
+        }
+    }
+}
+
+int run_noalloc(const Problem &pb, float *mi_out, hipStream_t stream) {
+    void *vbase = nullptr;
+    int rc = noalloc_base(&vbase);
+    if (rc) return rc;
+    char *base = (char *)vbase;
+    const int n = pb.nmax;
+    const size_t psi_off = 256;
+    const size_t rows_off = (psi_off + (size_t)(n + 1) * sizeof(float) + 255) & ~(size_t)255;
+    Config c = config_for(pb);
+    c.sym = 0;  // plain pass 2: one n-record row per wave
+    Shape s;
+    rc = shape_for(pb, c, pb.P, s);
+    if (rc) return rc;
+    const long long waves_fit =
+        (long long)((kNoAllocBytes - rows_off) / ((size_t)n * sizeof(KeyIdx)));
+    NSX_REQUIRE(waves_fit >= c.wpb, "nsx_ksg_mi_noalloc: the static scratch holds too few tie rows");
+    s.blocks = std::max<long long>(1, std::min<long long>(s.blocks, waves_fit / c.wpb));
+    Buffers b;
+    b.stats = (unsigned long long *)base;
+    b.win = (int *)(base + 64);
+    b.psi = (float *)(base + psi_off);
+    b.scratch = (KeyIdx *)(base + rows_off);
+    rc = fill_u32(b.stats, 0u, 4, stream);
+    if (!rc) rc = fill_u32(b.win, 0u, 1, stream);
+    if (!rc) rc = fill_u32(b.win + 1, (uint32_t)n, 1, stream);
+    if (!rc) rc = fill_u32(b.win + 2, 0u, 1, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(psi_table_kernel, dim3(1), dim3(256), 0, stream, b.psi, n);
+    NSX_HIP(hipGetLastError(), "psi table launch");
+    return launch(s, c, make_args(pb, c, b, mi_out, nullptr, pb.P), stream);
+}
+
 }  // namespace
 }  // namespace nsx
 
@@ -1171,6 +1261,25 @@ int nsx_ksg_mi_windows(const float *d_X, long long ld, long long dstride, int d,
     g_last_call_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return rc;
+}
+
+int nsx_ksg_mi_noalloc(const float *d_X, long long ld, long long dstride, int d, int n,
+                       const int *d_ab, long long P, int k, float *d_mi_out, void *stream) {
+    TraceRange trace("nsx_ksg_mi_noalloc");
+    clear_error();
+    NSX_REQUIRE(d_mi_out, "nsx_ksg_mi_noalloc: null output pointer");
+    NSX_REQUIRE(k <= ksg::kMaxK, "nsx_ksg_mi_noalloc: k must be at most 16");
+    NSX_REQUIRE(n >= 1 && n <= kNoAllocMaxFrames,
+                "nsx_ksg_mi_noalloc: n must be at most 262144 frames (the psi table and the "
+                "per-wave tie rows live in a 64 MiB static scratch)");
+    const int off = 0;
+    Problem pb;
+    int rc = make_problem(pb, d_X, ld, dstride, d, &off, &n, 1, d_ab, P, k);
+    if (rc) return rc;
+    rc = ensure_device();
+    if (rc) return rc;
+    if (P == 0) return kOk;
+    return run_noalloc(pb, d_mi_out, (hipStream_t)stream);
 }
 
 int nsx_ksg_last_tie_frames(long long *count) {
