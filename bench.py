@@ -329,11 +329,12 @@ def log(msg):
 def bench_bootstrap(args, abi, synth):
     """64 bootstrap replicates as contiguous windows (n_r ~ U[500, 4000]) of
     a 512-node (--boot-nodes), 4000-frame C-alpha-like trajectory, all pairs
-    (130,816 at 512 nodes) in each: one autotuned, hipGraph-captured
-    persistent all-frames launch; the windowed call on the sorted-list path
-    (window by window); and one all-frames launch per replicate (the
-    reference runs its pair loop once per replicate). All three must give
-    identical outputs."""
+    (130,816 at 512 nodes) in each. Timed: the hipGraph-captured plan on the
+    sorted-list path (one graph launch builds every window's tables and
+    walks all its pairs; no host synchronisation inside), the eager windowed
+    call on the same path, one call per replicate (the reference runs its
+    pair loop once per replicate), and the all-frames graph plan (autotuned
+    persistent launch). All four must give identical outputs."""
     N, nT, d, k, R = args.boot_nodes, 4000, 3, args.k, 64
     X, _ = synth.calpha_trajectory(N=N, n=nT, seed=7)
     ab = synth.all_pairs(N)
@@ -341,78 +342,85 @@ def bench_bootstrap(args, abi, synth):
     win = synth.bootstrap_windows(R=R, n_total=nT, lo=500, hi=4000, seed=3)
     dX = abi.DeviceArray.from_numpy(X)
     dab = abi.DeviceArray.from_numpy(ab)
-    out_b = abi.DeviceArray((R, P), np.float32)
-    out_s = abi.DeviceArray((R, P), np.float32)
-    work = float(sum(int(l) * (int(l) - 1) for _, l in win)) * P * (8 * d + 2)
     timer = abi.Timer()
+    outs = {}
 
-    log(f"bootstrap: {R} windows x {P} pairs, tuning")
+    def timed(name, fn, reps):
+        out = []
+        for _ in range(reps):
+            timer.start()
+            fn()
+            timer.stop()
+            out.append(timer.ms())
+            log(f"{name} {out[-1]:.1f} ms")
+        return float(np.min(out)), out
+
+    # List-path graph plan.
+    outs["graph_lists"] = abi.DeviceArray((R, P), np.float32)
     t0 = time.perf_counter()
-    plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, out_b, autotune=True)
-    tune_s = time.perf_counter() - t0
-    cfg = plan.config()
-    log(f"plan {cfg}")
+    plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, outs["graph_lists"],
+                       autotune=False)
+    create_s = time.perf_counter() - t0
+    assert plan.method() == 2
     plan.launch()
     abi.sync()
-    log("batched warmup done")
-    batched = []
-    for _ in range(args.steps):
-        timer.start()
-        plan.launch()
-        timer.stop()
-        batched.append(timer.ms())
-        log(f"batched replay {batched[-1]:.1f} ms")
+    t_graph, graph_ms = timed("list-path graph replay", plan.launch, args.steps)
     ties = plan.tie_frames()
+    plan.close()
 
-    # The windowed call on the sorted-list path: one window after another,
-    # each with its own tables.
-    out_l = abi.DeviceArray((R, P), np.float32)
-    abi.ksg_mi_windows(dX, X.shape[1], nT, d, win, dab, P, k, out_l)
-    abi.sync()
-    lists = []
-    for _ in range(args.steps):
-        timer.start()
-        abi.ksg_mi_windows(dX, X.shape[1], nT, d, win, dab, P, k, out_l)
-        timer.stop()
-        lists.append(timer.ms())
-        log(f"windows on the list path {lists[-1]:.1f} ms")
+    # Eager windows on the list path.
+    outs["eager_lists"] = abi.DeviceArray((R, P), np.float32)
+    run_w = lambda: abi.ksg_mi_windows(dX, X.shape[1], nT, d, win, dab, P, k,
+                                       outs["eager_lists"])
+    run_w()
+    t_eager, eager_ms = timed("eager windows (list path)", run_w, args.steps)
     lmethod, lentries = abi.last_method()
+    phases = abi.last_phase_ms()
 
-    # One all-frames launch per replicate (the reference's loop structure).
-    abi.set_method(abi.METHOD_ALL_FRAMES)
-    per_rep = []
-    for _ in range(1):
-        timer.start()
+    # One call per replicate (list path).
+    outs["per_replicate"] = abi.DeviceArray((R, P), np.float32)
+
+    def per_rep():
         for r, (off, ln) in enumerate(win):
-            abi.ksg_mi(dX, X.shape[1], nT, d, int(ln), dab, P, k, out_s,
+            abi.ksg_mi(dX, X.shape[1], nT, d, int(ln), dab, P, k, outs["per_replicate"],
                        frame_offset=int(off), out_offset=r * P)
-        timer.stop()
-        per_rep.append(timer.ms())
-        log(f"per-replicate launches {per_rep[-1]:.1f} ms")
+    t_rep, _ = timed("one call per replicate", per_rep, 1)
+
+    # All-frames graph plan (autotuned).
+    outs["graph_all_frames"] = abi.DeviceArray((R, P), np.float32)
+    abi.set_method(abi.METHOD_ALL_FRAMES)
+    t0 = time.perf_counter()
+    aplan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, outs["graph_all_frames"],
+                        autotune=True)
+    tune_s = time.perf_counter() - t0
     abi.set_method(abi.METHOD_AUTO)
-    same = bool(np.array_equal(out_b.numpy(), out_s.numpy()) and
-                np.array_equal(out_b.numpy(), out_l.numpy()))
-    tb, ts, tl = float(np.min(batched)), float(np.min(per_rep)), float(np.min(lists))
-    best = min(tb, tl)
+    cfg = aplan.config()
+    t_all, _ = timed("all-frames graph replay", aplan.launch, 1)
+    aplan.close()
+
+    ref = outs["graph_lists"].numpy()
+    same = all(np.array_equal(ref, o.numpy()) for o in outs.values())
+    work = float(sum(int(l) * (int(l) - 1) for _, l in win)) * P * (8 * d + 2)
     out = {
         "metric": "bootstrap replicate node-pairs/s",
-        "value": R * P / (best * 1e-3), "unit": "replicate node-pairs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": 1, "ms_per_step": best, "higher_is_better": True,
+        "value": R * P / (t_graph * 1e-3), "unit": "replicate node-pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": t_graph, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"bootstrap GC: {N} nodes, d={d}, k={k}, {R} windows "
-                               f"n_r ~ U[500,4000] of a {nT}-frame trajectory, {P} pairs each",
-                   "plan": cfg, "tune_s": tune_s},
-        "path": "sorted lists, window by window" if tl < tb else "all-frames graph launch",
-        "lists_windows_ms": tl, "lists_method": lmethod, "lists_entries": lentries,
-        "per_replicate_all_frames_ms": ts, "batched_graph_ms": tb,
-        "graph_speedup_vs_per_replicate": ts / tb, "lists_speedup_vs_graph": tb / tl,
+                               f"n_r ~ U[500,4000] of a {nT}-frame trajectory, {P} pairs each"},
+        "graph_lists_ms": t_graph, "graph_lists_steps_ms": graph_ms,
+        "graph_lists_create_s": create_s,
+        "eager_lists_ms": t_eager, "eager_lists_steps_ms": eager_ms,
+        "eager_lists_phases_ms": phases, "lists_entries": lentries,
+        "per_replicate_lists_ms": t_rep,
+        "graph_all_frames_ms": t_all, "graph_all_frames_plan": cfg, "tune_s": tune_s,
+        "graph_vs_per_replicate": t_rep / t_graph, "graph_vs_eager_windows": t_eager / t_graph,
+        "lists_vs_all_frames_graph": t_all / t_graph,
         "outputs_identical": same, "tie_frames": ties,
-        "roofline": {"bound": "valu", "achieved": work / (tb * 1e-3) / 1e12,
-                     "peak": VALU_PEAK / 1e12, "unit": "T lane-ops/s",
-                     "frac": work / (tb * 1e-3) / VALU_PEAK,
-                     "kernel": "ksg_kernel (graph launch): (8d+2)*n_r*(n_r-1) lane-ops per pair"},
+        "all_frames_equivalent": {"rate": work / (t_graph * 1e-3) / 1e12,
+                                  "unit": "T lane-ops/s",
+                                  "multiple_of_valu_peak": work / (t_graph * 1e-3) / VALU_PEAK},
     }
-    plan.close()
     print(json.dumps(out), flush=True)
 
 

@@ -174,9 +174,11 @@ int nsx_release_workspace(void);
 
 /* ---- batched plans: one hipGraph launch per replay ----------------------
  * A plan fixes X, the windows, the pairs and the output of
- * nsx_ksg_mi_windows, owns its psi table, window table and tie scratch, and
- * captures [tie-counter reset, persistent KSG launch] into a hipGraph, so a
- * replay is one hipGraphLaunch. autotune != 0 times candidate launch shapes
+ * nsx_ksg_mi_windows, owns every buffer its graph uses, and captures the
+ * whole pass into a hipGraph, so a replay is one hipGraphLaunch with no
+ * host synchronisation: on the sorted-list path (nsx_ksg_plan_method = 2)
+ * every window's table build, walk, tie resolution and psi sums; on the
+ * all-frames kernel [tie-counter reset, persistent KSG launch]. autotune != 0 times candidate launch shapes
  * (queries per lane x LDS frame tile x waves per workgroup) on a sample of
  * the items first and keeps the fastest (cached per (d, k, window-length
  * bucket); persisted to the file named by NSX_TUNE_CACHE when set).
@@ -193,6 +195,13 @@ int nsx_ksg_plan_tie_frames(nsx_ksg_plan plan, long long *count);
 int nsx_ksg_plan_config(nsx_ksg_plan plan, int *qpl, int *chunk, int *wpb,
                         long long *blocks, double *tune_ms, int *symmetric);
 int nsx_ksg_plan_destroy(nsx_ksg_plan plan);
+/* Algorithm a plan's graph runs: 2 = the sorted-list path (chosen, like
+ * nsx_ksg_set_method's automatic choice, when the windows have at most 8192
+ * frames whose two rows fit LDS and the coordinates are finite at creation;
+ * the graph then rebuilds every window's tables on each replay and carries
+ * the all-frames launch as a fallback that runs only if a coordinate has
+ * become non-finite), 1 = the all-frames kernel. */
+int nsx_ksg_plan_method(nsx_ksg_plan plan, int *method);
 
 /* GC transform of generalized_correlation_{cpu,gpu}.cpp:8-33 on host data:
  * h_gc[p] = MI <= 0 ? 0 : (float)sqrt(1 - (float)exp(-(2/(float)d)*MI)),

@@ -53,6 +53,7 @@ SIGNATURES = {
     "nsx_ksg_plan_tie_frames": [_VP, _LLP],
     "nsx_ksg_plan_config": [_VP, _IP, _IP, _IP, _LLP, ctypes.POINTER(ctypes.c_double), _IP],
     "nsx_ksg_plan_destroy": [_VP],
+    "nsx_ksg_plan_method": [_VP, _IP],
     "nsx_gc_from_mi_host": [_VP, _LL, _I, _VP],
     "nsx_row_mean": [_VP, _LL, _LL, _VP, _VP],
     "nsx_row_std": [_VP, _LL, _LL, _VP, _VP, _VP],
@@ -211,6 +212,12 @@ class KsgPlan:
               "plan config")
         return {"queries_per_lane": q.value, "frame_tile": c.value, "waves_per_workgroup": w.value,
                 "symmetric_pass2": sy.value, "workgroups": b.value, "tune_ms": t.value}
+
+    def method(self):
+        """2: the graph runs the sorted-list path; 1: the all-frames kernel."""
+        m = ctypes.c_int(0)
+        check(lib().nsx_ksg_plan_method(self.ptr, ctypes.byref(m)), "plan method")
+        return m.value
 
     def close(self):
         if self.ptr is not None and self.ptr.value:

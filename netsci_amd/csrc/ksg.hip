@@ -166,6 +166,7 @@ Args make_args(const Problem &pb, const Config &c, const Buffers &b, float *mi_o
     a.stats = b.stats;
     a.chunk = c.chunk;
     a.wpb = c.wpb;
+    a.run_if = nullptr;
     return a;
 }
 
@@ -646,16 +647,71 @@ void make_schedule(ListLayout &L, std::vector<int> &sched_of) {
     }
 }
 
+// Where list_prepare takes its device buffers: the library's grow-only
+// workspaces and table buffer (eager calls), or allocations owned by a plan
+// (a captured graph keeps its pointers, so nothing it uses may move).
+struct ListMem {
+    std::vector<void *> *owned = nullptr;  // null: workspaces
+
+    void *get(int slot, size_t bytes) {
+        if (!owned) return workspace(slot, bytes);
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("list path: device allocation failed");
+            return nullptr;
+        }
+        owned->push_back(p);
+        return p;
+    }
+
+    // Table slots: at most want, at least min (else *slots < min).
+    int tables(size_t slot_bytes, long long want, long long min, void **ptr, long long *slots) {
+        if (!owned) return table_space(slot_bytes, want, min, ptr, slots);
+        *ptr = nullptr;
+        *slots = 0;
+        size_t free_b = 0, total_b = 0;
+        NSX_HIP(hipMemGetInfo(&free_b, &total_b), "memory info");
+        double budget = 0.85 * (double)free_b - (double)(1ull << 30);
+        if (table_budget()) budget = std::min(budget, (double)table_budget());
+        long long s = budget > 0 ? std::min<long long>(want, (long long)(budget / (double)slot_bytes)) : 0;
+        while (s >= min && s > 0) {
+            void *p = nullptr;
+            if (hipMalloc(&p, (size_t)s * slot_bytes + 256) == hipSuccess) {
+                owned->push_back(p);
+                *ptr = p;
+                *slots = s;
+                return kOk;
+            }
+            (void)hipGetLastError();
+            s = s * 3 / 4;
+        }
+        return kOk;
+    }
+};
+
+// Every window's node spans and the non-finite flag (flags[0]) of the used
+// nodes; flags must be cleared first.
+int enqueue_node_stats(const Problem &pb, const ListLayout &L, hipStream_t stream) {
+    for (int w = 0; w < pb.R; ++w) {
+        hipLaunchKernelGGL(node_stats_kernel, dim3((unsigned)L.N), dim3(64), 0, stream,
+                           pb.X + pb.host_win[w], pb.ld, pb.dstride, pb.d, pb.host_win[pb.R + w],
+                           L.node_of, L.count, L.span + (long long)w * L.N, L.flags);
+        NSX_HIP(hipGetLastError(), "node stats launch");
+    }
+    return kOk;
+}
+
 // Host-side sizing with device readbacks (not capturable): the node map,
 // every window's spans and the finite check, the table slots, the
 // schedule, and for blocked tables the pair permutation. *ok = false: the
 // list path does not apply (non-finite coordinates or no room for tables).
-int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok) {
+int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok, ListMem mem) {
     *ok = false;
     const long long m2 = 2 * pb.P;
     // flags: [0] min node, [1] max node (first), then reused as described
     // in ListLayout.
-    unsigned *flags = (unsigned *)workspace(kWsListStats, 256);
+    unsigned *flags = (unsigned *)mem.get(kWsListStats, 256);
     if (!flags) return kHipError;
     int *mm = (int *)flags;
     int rc = fill_u32(mm, 0x7fffffffu, 1, stream);
@@ -672,8 +728,8 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok)
     NSX_REQUIRE(h_mm[0] >= 0, "ksg: negative node index in ab");
     const int N = h_mm[1] + 1;
     L.N = N;
-    int *map = (int *)workspace(kWsListMap, ((size_t)4 * N + 16) * sizeof(int));
-    L.span = (float *)workspace(kWsListSpan, (size_t)pb.R * N * sizeof(float));
+    int *map = (int *)mem.get(kWsListMap, ((size_t)4 * N + 16) * sizeof(int));
+    L.span = (float *)mem.get(kWsListSpan, (size_t)pb.R * N * sizeof(float));
     if (!map || !L.span) return kHipError;
     L.used = map;
     L.slot_of = map + N;
@@ -690,12 +746,8 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok)
     hipLaunchKernelGGL(slot_scan_kernel, dim3(1), dim3(1024), 0, stream, L.used, N, L.slot_of,
                        L.node_of, L.count);
     NSX_HIP(hipGetLastError(), "slot scan launch");
-    for (int w = 0; w < pb.R; ++w) {
-        hipLaunchKernelGGL(node_stats_kernel, dim3((unsigned)N), dim3(64), 0, stream,
-                           pb.X + pb.host_win[w], pb.ld, pb.dstride, pb.d, pb.host_win[pb.R + w],
-                           L.node_of, L.count, L.span + (long long)w * N, flags);
-        NSX_HIP(hipGetLastError(), "node stats launch");
-    }
+    rc = enqueue_node_stats(pb, L, stream);
+    if (rc) return rc;
     int h_u[2] = {0, 0};
     NSX_HIP(hipMemcpyAsync(&h_u[0], L.count, sizeof(int), hipMemcpyDeviceToHost, stream),
             "slot count readback");
@@ -722,7 +774,7 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok)
         // Table slots: all used nodes when they fit, else at least two.
         void *tab = nullptr;
         const size_t sb = slot_bytes_for(pb.nmax);
-        rc = table_space(sb, L.U, std::min(L.U, 2), &tab, &L.S);
+        rc = mem.tables(sb, L.U, std::min(L.U, 2), &tab, &L.S);
         if (rc) return rc;
         if (L.S < std::min(L.U, 2)) {
             NSX_REQUIRE(g_method != 2 && pb.k <= ksg::kMaxK,
@@ -744,7 +796,7 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok)
         const int nb = L.nb;
         const int nwalks = nb * (nb + 1) / 2;
         const size_t perm_words = (size_t)pb.P * 4 + (size_t)nb * nb + 2 * (size_t)nwalks + 64;
-        int *pw = (int *)workspace(kWsListPerm, perm_words * sizeof(int));
+        int *pw = (int *)mem.get(kWsListPerm, perm_words * sizeof(int));
         if (!pw) return kHipError;
         int *ab_perm = pw, *idx_perm = pw + 2 * pb.P, *key = pw + 3 * pb.P;
         int *d_sched = pw + 4 * pb.P;
@@ -817,7 +869,7 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok)
         1, std::min<long long>(pb.P, (1LL << 32) / ((long long)pb.nmax * (long long)sizeof(unsigned))));
     L.tie_cap = (unsigned)std::max<long long>(1024, L.chunk * (long long)pb.nmax / 32);
     const size_t cnt_words = (size_t)L.chunk * pb.nmax + pb.nmax;
-    L.cnt = (unsigned *)workspace(kWsListCnt, (cnt_words + L.tie_cap) * sizeof(unsigned));
+    L.cnt = (unsigned *)mem.get(kWsListCnt, (cnt_words + L.tie_cap) * sizeof(unsigned));
     if (!L.cnt) return kHipError;
     L.tie_list = L.cnt + cnt_words;
     *ok = true;
@@ -1030,7 +1082,7 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
 
     ListLayout L;
     bool ok = false;
-    rc = list_prepare(pb, L, stream, &ok);
+    rc = list_prepare(pb, L, stream, &ok, ListMem{});
     if (rc || !ok) return rc;
     hipEvent_t ev_prep = pm.ev();
     if (ev_prep) (void)hipEventRecord(ev_prep, stream);
@@ -1212,6 +1264,9 @@ struct nsx_ksg_plan_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     double tune_ms = 0;
+    int method = 1;              // 1 all-frames graph, 2 list-path graph
+    nsx::ListLayout L;           // list-path graph: layout over `owned`
+    std::vector<void *> owned;   // list-path graph buffers
 };
 
 using namespace nsx;
@@ -1226,6 +1281,7 @@ void destroy_plan(nsx_ksg_plan_s *p) {
     (void)hipFree(p->buf.win);
     (void)hipFree(p->buf.scratch);
     (void)hipFree(p->buf.stats);
+    for (void *q : p->owned) (void)hipFree(q);
     delete p;
 }
 
@@ -1385,7 +1441,30 @@ int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long
         destroy_plan(p);
         return kHipError;
     }
-    if (autotune) {
+    // The list path when it applies (windows of at most 8192 frames whose
+    // pair records fit LDS; method 0 or 2): sized now, with plan-owned
+    // buffers, from this plan's fixed pairs.
+    if (g_method != 1) {
+        const size_t rec = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+        const bool eligible = pb.nmax <= ksgw::kMaxFrames && rec + 256 <= 160 * 1024;
+        if (!eligible && g_method == 2) {
+            set_error("nsx_ksg_plan_create: the list path needs windows of at most 8192 frames "
+                      "whose two rows fit in LDS");
+            rc = kInvalidArgument;
+        }
+        if (!rc && eligible) {
+            bool ok = false;
+            rc = list_prepare(pb, p->L, cs, &ok, ListMem{&p->owned});
+            if (!rc && ok) p->method = 2;
+        }
+        if (rc) {
+            (void)hipStreamSynchronize(cs);
+            (void)hipStreamDestroy(cs);
+            destroy_plan(p);
+            return rc;
+        }
+    }
+    if (autotune && p->method == 1) {
         Config best;
         rc = nsx::autotune(pb, p->buf, cs, best, &p->tune_ms);
         if (rc) {
@@ -1406,14 +1485,25 @@ int nsx_ksg_plan_create(nsx_ksg_plan *plan, const float *d_X, long long ld, long
         set_error("nsx_ksg_plan_create: tie scratch too small for the chosen shape");
         rc = kInternal;
     }
-    // Capture: reset the tie counter (a fill kernel, not a memset node), then
-    // the persistent launch.
+    WalkShape ws;
+    if (!rc && p->method == 2) rc = walk_shape(pb, ws);
+    // Capture. All-frames: reset the tie counter (a fill kernel, not a
+    // memset node), then the persistent launch. List path: clear the flags,
+    // the spans and finite check of every window, every window's builds,
+    // walks, ties and psi sums, then the all-frames launch guarded by the
+    // non-finite flag (it returns at once unless a coordinate is not finite).
     if (!rc) {
         e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
-            rc = fill_u32(p->buf.stats, 0u, 2, cs);
-            if (!rc) rc = launch(p->shape, p->cfg,
-                                 make_args(pb, p->cfg, p->buf, d_mi_out, nullptr, pb.P), cs);
+            if (p->method == 2) {
+                rc = fill_u32(p->L.flags, 0u, 32, cs);
+                if (!rc) rc = enqueue_node_stats(pb, p->L, cs);
+                if (!rc) rc = list_enqueue(pb, p->L, ws, p->buf.psi, d_mi_out, nullptr, cs, nullptr);
+            }
+            if (!rc) rc = fill_u32(p->buf.stats, 0u, 2, cs);
+            Args fa = make_args(pb, p->cfg, p->buf, d_mi_out, nullptr, pb.P);
+            if (p->method == 2) fa.run_if = p->L.flags;
+            if (!rc) rc = launch(p->shape, p->cfg, fa, cs);
             hipError_t e2 = hipStreamEndCapture(cs, &p->graph);
             if (!rc && e2 != hipSuccess) {
                 set_error(std::string("end capture: ") + hipGetErrorString(e2));
@@ -1452,10 +1542,20 @@ int nsx_ksg_plan_launch(nsx_ksg_plan plan, void *stream) {
 int nsx_ksg_plan_tie_frames(nsx_ksg_plan plan, long long *count) {
     clear_error();
     NSX_REQUIRE(plan && count, "nsx_ksg_plan_tie_frames: invalid arguments");
-    unsigned long long v = 0;
+    unsigned long long v = 0, w = 0;
     NSX_HIP(hipDeviceSynchronize(), "plan sync");
     NSX_HIP(hipMemcpy(&v, plan->buf.stats, sizeof(v), hipMemcpyDeviceToHost), "tie readback");
-    *count = (long long)v;
+    if (plan->method == 2)
+        NSX_HIP(hipMemcpy(&w, plan->L.flags + 8, sizeof(w), hipMemcpyDeviceToHost),
+                "tie readback");
+    *count = (long long)(v + w);
+    return kOk;
+}
+
+int nsx_ksg_plan_method(nsx_ksg_plan plan, int *method) {
+    clear_error();
+    NSX_REQUIRE(plan && method, "nsx_ksg_plan_method: invalid arguments");
+    *method = plan->method;
     return kOk;
 }
 

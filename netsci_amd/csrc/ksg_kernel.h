@@ -63,6 +63,9 @@ struct Args {
     unsigned long long *stats;  // [0] tie frames
     int chunk;           // frames per LDS tile: 64, 128 or 256
     int wpb;             // waves per workgroup
+    const unsigned *run_if;  // nullable: the launch does nothing unless *run_if != 0
+                             // (the fallback a list-path graph carries for
+                             // non-finite coordinates)
 };
 
 using KernelFn = void (*)(Args);
@@ -456,6 +459,7 @@ __device__ __forceinline__ void fence_agent() {
 template <int D, int K, int QPL, bool kSym>
 __global__ __launch_bounds__(kWave *kMaxWavesPerBlock) void ksg_kernel(Args a) {
     extern __shared__ __attribute__((aligned(16))) float dyn_lds[];
+    if (a.run_if && *a.run_if == 0u) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int chunk = a.chunk;

@@ -138,7 +138,7 @@ using TieFn = void (*)(WalkArgs);
 
 // LDS bytes of the two kernels.
 inline size_t build_lds_bytes(int d, int n, long long rs) {
-    if (n > 2048) return (size_t)kBuildWaves * 2048 * 8;  // the wide kernel's exchange rows
+    if (n > 2048) return (size_t)kBuildWaves * 2048 * 4;  // the wide kernel's exchange rows
     return (((size_t)d * n * 4 + 15) & ~(size_t)15) + (size_t)kBuildWaves * rs * 2;
 }
 inline size_t walk_lds_bytes(int d, int n, int nbuf) {
@@ -223,94 +223,6 @@ __device__ __forceinline__ unsigned xor_lane(unsigned v) {
         return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (M << 10));
     } else {
         return (unsigned)__shfl_xor((int)v, M);
-    }
-}
-
-template <int M>
-__device__ __forceinline__ unsigned long long xor_lane_u64(unsigned long long v) {
-    const unsigned lo = xor_lane<M>((unsigned)v);
-    const unsigned hi = xor_lane<M>((unsigned)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
-}
-
-// One cross-lane bitonic stage: partner lane = lane ^ LX. ebase is the
-// element index of the wave's first element (its 2048-element block in the
-// row), so the merge direction is that of the whole row's sort.
-template <int R, int LX>
-__device__ __forceinline__ void bitonic_cross(unsigned long long (&v)[R], int lane, int size,
-                                              int ebase) {
-    const bool lower = (lane & LX) == 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const bool asc = ((ebase + lane * R + r) & size) == 0;
-        const unsigned long long o = xor_lane_u64<LX>(v[r]);
-        const bool keep = (v[r] < o) == (lower == asc);
-        v[r] = keep ? v[r] : o;
-    }
-}
-
-// One bitonic stage (merge size SIZE, compare distance STRIDE) and the
-// stages after it, unrolled at compile time; element e = lane * R + r.
-template <int R, int SIZE, int STRIDE>
-__device__ __forceinline__ void bitonic_stages(unsigned long long (&v)[R], int lane,
-                                               int ebase = 0) {
-    if constexpr (STRIDE >= R) {
-        bitonic_cross<R, STRIDE / R>(v, lane, SIZE, ebase);
-    } else {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r & STRIDE) continue;
-            const int r2 = r | STRIDE;
-            const bool asc = ((ebase + lane * R + r) & SIZE) == 0;
-            const unsigned long long x = v[r], y = v[r2];
-            const bool sw = asc ? (x > y) : (x < y);
-            v[r] = sw ? y : x;
-            v[r2] = sw ? x : y;
-        }
-    }
-    if constexpr (STRIDE > 1) bitonic_stages<R, SIZE, STRIDE / 2>(v, lane, ebase);
-}
-
-template <int R, int SIZE>
-__device__ __forceinline__ void bitonic_sizes(unsigned long long (&v)[R], int lane,
-                                              int ebase = 0) {
-    bitonic_stages<R, SIZE, SIZE / 2>(v, lane, ebase);
-    if constexpr (SIZE < kWave * R) bitonic_sizes<R, SIZE * 2>(v, lane, ebase);
-}
-
-// Ascending bitonic sort of 64 * R distinct keys; element e = lane * R + r.
-// With ebase != 0 the wave sorts its block in the direction a row-wide sort
-// gives it (descending when bit log2(64 R) of ebase is set).
-template <int R>
-__device__ __forceinline__ void bitonic_sort(unsigned long long (&v)[R], int lane,
-                                             int ebase = 0) {
-    bitonic_sizes<R, 2>(v, lane, ebase);
-}
-
-// Row-wide merges after every wave sorted its 2048-element block: sizes
-// 4096 .. 2048 W. Strides of 2048 and more exchange through LDS (xr: this
-// row's W x 2048 elements); smaller ones stay in the wave.
-template <int W, int SIZE>
-__device__ __forceinline__ void wide_merges(unsigned long long (&v)[32], int lane, int ebase,
-                                            unsigned long long *xr) {
-    if constexpr (SIZE <= 2048 * W) {
-#pragma unroll
-        for (int stride = SIZE / 2; stride >= 2048; stride >>= 1) {
-#pragma unroll
-            for (int r = 0; r < 32; ++r) xr[ebase + lane * 32 + r] = v[r];
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                const int e = ebase + lane * 32 + r;
-                const unsigned long long o = xr[e ^ stride];
-                const bool asc = (e & SIZE) == 0;
-                const bool keep = (v[r] < o) == (((e & stride) == 0) == asc);
-                v[r] = keep ? v[r] : o;
-            }
-            __syncthreads();
-        }
-        bitonic_stages<32, SIZE, 1024>(v, lane, ebase);
-        wide_merges<W, SIZE * 2>(v, lane, ebase, xr);
     }
 }
 
@@ -582,16 +494,60 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
     }
 }
 
+// Cross-wave stages of the all-ascending merges of a row sorted by W waves
+// (2048 32-bit values each, element e = lane * 32 + r of wave wl). The LDS
+// copy of a wave's block is stored as [r][lane], so both the writes and the
+// mirrored reads cover 64 consecutive words.
+__device__ __forceinline__ void wave_exchange(unsigned (&v)[32], int lane, int wl, int pw,
+                                              bool mirror, unsigned *xr) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) xr[wl * 2048 + r * kWave + lane] = v[r];
+    __syncthreads();
+    const bool lower = wl < pw;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        const unsigned o = mirror ? xr[pw * 2048 + (31 - r) * kWave + (kWave - 1 - lane)]
+                                  : xr[pw * 2048 + r * kWave + lane];
+        v[r] = lower ? min(v[r], o) : max(v[r], o);
+    }
+    __syncthreads();
+}
+
+// Merges of sizes 4096 and (W = 4) 8192 after each wave sorted its block:
+// the mirror stage pairs local e of one wave with local 2047 - e of its
+// partner, the stride-2048 stage pairs equal positions, and the strides
+// 1024 .. 1 stay within the wave.
+template <int W>
+__device__ __forceinline__ void wide_merges32(unsigned (&v)[32], int lane, int wl, unsigned *xr) {
+    wave_exchange(v, lane, wl, wl ^ 1, true, xr);
+    asc_stages<32, 1024>(v, lane);
+    if constexpr (W == 4) {
+        wave_exchange(v, lane, wl, 3 - wl, true, xr);
+        wave_exchange(v, lane, wl, wl ^ 1, false, xr);
+        asc_stages<32, 1024>(v, lane);
+    }
+}
+
 // Rows of 2048 < n <= 2048 W frames: W waves sort one row together (each a
-// 2048-element block, 32 per lane), merging across waves through LDS. A
-// workgroup of kBuildWaves waves holds kBuildWaves / W rows; coordinates are
-// read from global memory (the node's row stays in L2).
+// 2048-element block, 32 per lane) as 32-bit values (key bits >> 12) << 13
+// | j: every in-wave compare-exchange is a v_min_u32 / v_max_u32 with DPP
+// or permlane partners, as in list_build_kernel, and the cross-wave stages
+// go through LDS. The truncated key t = value >> 13 keeps the order of the
+// non-negative keys (equal truncations only within 2^-11 relative) and
+// j < 8192 fits the low bits. Only elements that share t with a neighbour
+// need their exact key bits (ek): odd-even transposition on (ek, j) inside
+// those runs, across the row's waves, restores exact (key, j) order, and
+// the upper-bound ranks come from the ends of runs of equal (t, ek). A
+// workgroup of kBuildWaves waves holds kBuildWaves / W rows of one node;
+// coordinates are read from global memory (the node's row stays in cache).
 template <int D, int W>
-__global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(BuildArgs a) {
-    extern __shared__ __attribute__((aligned(16))) unsigned long long xch[];
-    __shared__ unsigned first_key[kBuildWaves];
+__global__ __launch_bounds__(kWave *kBuildWaves, 2) void list_build_wide_kernel(BuildArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned xch[];
+    __shared__ unsigned edge_u[2][kBuildWaves], edge_k[2][kBuildWaves];  // [0] first, [1] last
     __shared__ unsigned wave_min[kBuildWaves];
+    __shared__ int any_swap;
     constexpr int kRows = kBuildWaves / W;
+    constexpr unsigned kPad = 0xffffffffu;  // above every packed finite key
     const int n = a.n;
     const int slot = blockIdx.x / a.groups;
     const int g = blockIdx.x - slot * a.groups;
@@ -601,7 +557,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
     const int rg = wib / W, wl = wib - rg * W;  // row in the workgroup, block in the row
     const int ebase = wl * 2048;
     if (a.abort && *a.abort) return;
-    unsigned long long *xr = xch + rg * (W * 2048);
+    unsigned *xr = xch + rg * (W * 2048);
     uint16_t *st = reinterpret_cast<uint16_t *>(xr);  // rank staging after the sort
     for (int base = g * kRows; base < n; base += a.groups * kRows) {
         const int i = base + rg;
@@ -611,7 +567,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         for (int d = 0; d < D; ++d) q[d] = row ? x[d * a.dstride + i] : 0.0f;
         // Any initial placement sorts the same: frame ebase + r * 64 + lane
         // starts in register r of this lane, so each load is coalesced.
-        unsigned long long v[32];
+        unsigned u[32];
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
             const int e = ebase + r * kWave + lane;
@@ -619,14 +575,105 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
                 float c[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) c[d] = x[d * a.dstride + e];
-                const float key = marg<D>(q, c);
-                v[r] = ((unsigned long long)__float_as_uint(key) << 32) | (unsigned)e;
+                u[r] = ((__float_as_uint(marg<D>(q, c)) >> 12) << 13) | (unsigned)e;
             } else {
-                v[r] = ~0ull;
+                u[r] = kPad;
             }
+            // At most 8 elements' loads in flight (the compiler would issue
+            // all 96 first and run out of registers).
+            if ((r & 7) == 7) asm volatile("" ::: "memory");
         }
-        bitonic_sort<32>(v, lane, ebase);
-        wide_merges<W, 4096>(v, lane, ebase, xr);
+        mirror_merges<32, 2>(u, lane);
+        wide_merges32<W>(u, lane, wl, xr);
+
+        // Neighbours across lanes and waves: (lane, 31) -> (lane + 1, 0),
+        // the wave's last -> the next wave's first (kPad past the row).
+        if (lane == 0) edge_u[0][wib] = u[0];
+        if (lane == kWave - 1) edge_u[1][wib] = u[31];
+        __syncthreads();
+        unsigned un = (unsigned)__shfl_down((int)u[0], 1);
+        unsigned up = (unsigned)__shfl_up((int)u[31], 1);
+        if (lane == kWave - 1) un = wl + 1 < W ? edge_u[0][wib + 1] : kPad;
+        if (lane == 0) up = wl > 0 ? edge_u[1][wib - 1] : kPad;
+        __syncthreads();
+        // Exact key bits of the elements in runs of equal truncation.
+        unsigned ek[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            const unsigned t = u[r] >> 13;
+            const unsigned tp = (r == 0 ? up : u[r - 1]) >> 13;
+            const unsigned tn = (r == 31 ? un : u[r + 1]) >> 13;
+            ek[r] = 0u;
+            if (u[r] != kPad && (t == tp || t == tn)) {
+                const int j = (int)(u[r] & 0x1fffu);
+                float c[D];
+#pragma unroll
+                for (int d = 0; d < D; ++d) c[d] = x[d * a.dstride + j];
+                ek[r] = __float_as_uint(marg<D>(q, c));
+            }
+            if ((r & 7) == 7) asm volatile("" ::: "memory");
+        }
+        // Odd-even transposition within runs, until no pair of the
+        // workgroup's rows swaps. A pair with different truncations is in
+        // order already.
+        auto cx = [](unsigned &ux, unsigned &kx, unsigned &uy, unsigned &ky) {
+            const bool sw = (ux >> 13) == (uy >> 13) &&
+                            (kx > ky || (kx == ky && (ux & 0x1fffu) > (uy & 0x1fffu)));
+            const unsigned tu = ux, tk = kx;
+            ux = sw ? uy : ux;
+            kx = sw ? ky : kx;
+            uy = sw ? tu : uy;
+            ky = sw ? tk : ky;
+            return sw;
+        };
+        for (;;) {
+            bool swapped = false;
+#pragma unroll
+            for (int r = 0; r + 1 < 32; r += 2) swapped |= cx(u[r], ek[r], u[r + 1], ek[r + 1]);
+#pragma unroll
+            for (int r = 1; r + 1 < 32; r += 2) swapped |= cx(u[r], ek[r], u[r + 1], ek[r + 1]);
+            if (threadIdx.x == 0) any_swap = 0;
+            if (lane == 0) {
+                edge_u[0][wib] = u[0];
+                edge_k[0][wib] = ek[0];
+            }
+            if (lane == kWave - 1) {
+                edge_u[1][wib] = u[31];
+                edge_k[1][wib] = ek[31];
+            }
+            unsigned nu = (unsigned)__shfl_down((int)u[0], 1);
+            unsigned nk = (unsigned)__shfl_down((int)ek[0], 1);
+            unsigned pu = (unsigned)__shfl_up((int)u[31], 1);
+            unsigned pk = (unsigned)__shfl_up((int)ek[31], 1);
+            __syncthreads();
+            if (lane == kWave - 1) {
+                nu = wl + 1 < W ? edge_u[0][wib + 1] : kPad;
+                nk = wl + 1 < W ? edge_k[0][wib + 1] : 0u;
+            }
+            if (lane == 0) {
+                pu = wl > 0 ? edge_u[1][wib - 1] : 0u;
+                pk = wl > 0 ? edge_k[1][wib - 1] : 0u;
+            }
+            // (lane, 31) keeps the smaller of itself and its successor,
+            // (lane, 0) the larger of its predecessor and itself.
+            {
+                unsigned au = u[31], ak = ek[31], bu = nu, bk = nk;
+                swapped |= cx(au, ak, bu, bk);
+                u[31] = au;
+                ek[31] = ak;
+            }
+            {
+                unsigned au = pu, ak = pk, bu = u[0], bk = ek[0];
+                cx(au, ak, bu, bk);
+                u[0] = bu;
+                ek[0] = bk;
+            }
+            if (__ballot(swapped) && lane == 0) atomicOr(&any_swap, 1);
+            __syncthreads();
+            const int again = any_swap;
+            __syncthreads();
+            if (!again) break;
+        }
 
         // ord row.
         const long long orow = table_row(slot, row ? i : 0, n, a.rs);
@@ -638,8 +685,8 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
                 // walk reads whole blocks and never inserts frame i.
                 unsigned e8[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    e8[u] = e0 + u < n ? (unsigned)(v[r0 + u] & 0xffff) : (unsigned)i;
+                for (int v = 0; v < 8; ++v)
+                    e8[v] = e0 + v < n ? (u[r0 + v] & 0x1fffu) : (unsigned)i;
                 uint4 w;
                 w.x = e8[0] | (e8[1] << 16);
                 w.y = e8[2] | (e8[3] << 16);
@@ -649,24 +696,32 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
             }
         }
 
-        // Upper-bound ranks: the first end of an equal-key run at or after
-        // each position, across the row's waves.
-        if (lane == 0) first_key[wib] = (unsigned)(v[0] >> 32);
+        // Upper-bound ranks: the first end of a run of equal exact keys at or
+        // after each position, across the row's waves. Position e ends a run
+        // when its successor has another truncation or exact key.
+        if (lane == 0) {
+            edge_u[0][wib] = u[0];
+            edge_k[0][wib] = ek[0];
+        }
         __syncthreads();
         constexpr unsigned kBig = 0x7fffffffu;
-        const unsigned next_wave_first = wl + 1 < W ? first_key[wib + 1] : 0xffffffffu;
-        const unsigned next0 = (unsigned)__shfl_down((int)(unsigned)(v[0] >> 32), 1);
-        unsigned sfx[32];
+        unsigned su = (unsigned)__shfl_down((int)u[0], 1);
+        unsigned sk = (unsigned)__shfl_down((int)ek[0], 1);
+        if (lane == kWave - 1) {
+            su = wl + 1 < W ? edge_u[0][wib + 1] : kPad;
+            sk = wl + 1 < W ? edge_k[0][wib + 1] : 0u;
+        }
+        // The lane's first run end, then the same scan again below while
+        // the ranks are written (no per-position array).
+        auto ends_run = [&](int r) {
+            const unsigned nu2 = r == 31 ? su : u[r + 1];
+            const unsigned nk2 = r == 31 ? sk : ek[r + 1];
+            return (nu2 >> 13) != (u[r] >> 13) || nk2 != ek[r];
+        };
         unsigned m = kBig;
 #pragma unroll
-        for (int r = 31; r >= 0; --r) {
-            const unsigned kr = (unsigned)(v[r] >> 32);
-            const unsigned kn = (r == 31) ? (lane == kWave - 1 ? next_wave_first : next0)
-                                          : (unsigned)(v[r + 1] >> 32);
-            const unsigned end = kn != kr ? (unsigned)(ebase + lane * 32 + r + 1) : kBig;
-            m = min(m, end);
-            sfx[r] = m;
-        }
+        for (int r = 31; r >= 0; --r)
+            m = ends_run(r) ? (unsigned)(ebase + lane * 32 + r + 1) : m;
         unsigned t = m;
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
@@ -680,10 +735,12 @@ __global__ __launch_bounds__(kWave *kBuildWaves) void list_build_wide_kernel(Bui
         unsigned after = (unsigned)__shfl_down((int)t, 1);
         after = lane == kWave - 1 ? later : min(after, later);
         if (row) {
+            unsigned cur = after;
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
+            for (int r = 31; r >= 0; --r) {
                 const int e = ebase + lane * 32 + r;
-                if (e < n) st[(unsigned)(v[r] & 0xffff)] = (uint16_t)min(sfx[r], after);
+                cur = ends_run(r) ? (unsigned)(e + 1) : cur;
+                if (e < n) st[u[r] & 0x1fffu] = (uint16_t)cur;
             }
         }
         __syncthreads();

@@ -71,6 +71,7 @@ void *workspace(int slot, size_t bytes);
 int table_space(size_t slot_bytes, long long want_slots, long long min_slots, void **ptr,
                 long long *slots);
 void set_table_budget(size_t bytes);
+size_t table_budget();  // 0: no cap set
 // Frees every workspace and the table buffer (the device must be idle).
 void release_workspaces();
 

@@ -74,6 +74,11 @@ void set_table_budget(size_t bytes) {
     g_tab_budget = bytes;
 }
 
+size_t table_budget() {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    return g_tab_budget;
+}
+
 int table_space(size_t slot_bytes, long long want_slots, long long min_slots, void **ptr,
                 long long *slots) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);

@@ -14,9 +14,10 @@
   symmetry to rounding; 1,000 seeded decoded paths are edge walks whose
   weights sum to H).
 * C3: 512 nodes, 64 windows n_r ~ U[500, 4000] of a 4000-frame trajectory:
-  the graph-captured all-frames plan, the windowed list-path call and one
-  list-path call per window agree bitwise on all 64 x 130,816 items, and 16
-  seeded (window, pair) items equal the oracle.
+  the graph-captured all-frames plan, the graph-captured list-path plan
+  (two replays), the windowed list-path call and one list-path call per
+  window agree bitwise on all 64 x 130,816 items, and 16 seeded (window,
+  pair) items equal the oracle.
 * C4: N=4000, d=3, n=5000, k=6, all 7,998,000 pairs: the plain pass, then
   the screened pass (screener trained on 8 other trajectories): recall of
   GC > 0.3 >= 0.995, every computed value bit-identical to the plain pass,
@@ -243,12 +244,27 @@ def test_c3_bootstrap_plan_lists_per_window():
     win = synth.bootstrap_windows(R=R, n_total=nT, lo=500, hi=4000, seed=3)
     dX = abi.DeviceArray.from_numpy(X)
     dab = abi.DeviceArray.from_numpy(ab)
+    # Two graph plans: the all-frames kernel (every frame pair) and the
+    # sorted-list path, independent algorithms.
     out_plan = abi.DeviceArray((R, P), np.float32)
+    abi.set_method(abi.METHOD_ALL_FRAMES)
     plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, out_plan, autotune=False)
+    abi.set_method(abi.METHOD_AUTO)
+    assert plan.method() == 1
     plan.launch()
     abi.sync()
     got_plan = out_plan.numpy()
     plan.close()
+    _flush("C3: list-path graph plan")
+    out_lp = abi.DeviceArray((R, P), np.float32)
+    plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, out_lp, autotune=False)
+    assert plan.method() == 2
+    for _ in range(2):
+        plan.launch()
+        abi.sync()
+        np.testing.assert_array_equal(out_lp.numpy(), got_plan)
+    plan.close()
+    del out_lp
     _flush("C3: windows on the list path")
     out_l = abi.DeviceArray((R, P), np.float32)
     abi.ksg_mi_windows(dX, X.shape[1], nT, d, win, dab, P, k, out_l)
