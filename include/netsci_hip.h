@@ -111,6 +111,22 @@ int nsx_ksg_mi_noalloc(const float *d_X, long long ld, long long dstride, int d,
                        int n, const int *d_ab, long long P, int k, float *d_mi_out,
                        void *stream);
 
+/* The pair loop in chunks, for checkpointing callers: the reference's
+ * checkpointed pair loops (netcalc/src/generalized_correlation.cpp:8-101,
+ * mutual_information.cpp:8-101) save R after every checkpointFrequency
+ * pairs. Same results as nsx_ksg_mi over all P pairs, computed in chunks
+ * [h_ends[c-1], h_ends[c]) (h_ends strictly increasing, the last equal to P)
+ * with the list path's tables built once. When chunk c is done, its MI
+ * values are in d_mi_out and h_mi_out[h_ends[c-1] .. h_ends[c]), and
+ * on_chunk(user, h_ends[c]) runs on the calling thread while the device
+ * works on the next chunk; a non-zero return stops the call with an error.
+ * Returns when every chunk is done. */
+typedef int (*nsx_chunk_fn)(void *user, long long end);
+int nsx_ksg_mi_chunked(const float *d_X, long long ld, long long dstride, int d,
+                       int n, const int *d_ab, long long P, int k, float *d_mi_out,
+                       float *h_mi_out, const long long *h_ends, int nends,
+                       nsx_chunk_fn on_chunk, void *user, void *stream);
+
 /* Frames whose k-th / (k+1)-th joint distances tied in the last KSG launch
  * on this thread (resolved on the device in libstdc++ std::sort order). */
 int nsx_ksg_last_tie_frames(long long *count);

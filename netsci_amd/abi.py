@@ -38,6 +38,8 @@ SIGNATURES = {
     "nsx_ksg_mi": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP, _VP],
     "nsx_ksg_mi_windows": [_VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I, _VP, _VP],
     "nsx_ksg_mi_noalloc": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP],
+    "nsx_ksg_mi_chunked": [_VP, _LL, _LL, _I, _I, _VP, _LL, _I, _VP, _VP, _LLP, _I, _VP, _VP,
+                           _VP],
     "nsx_ksg_last_tie_frames": [_LLP],
     "nsx_ksg_set_config": [_I, _I, _I, _I, _I],
     "nsx_ksg_set_method": [_I, _I, _I],

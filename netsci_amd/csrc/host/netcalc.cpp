@@ -91,6 +91,9 @@ int pair_loop(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int
     return platform;
 }
 
+void gpu_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n, int d,
+              bool gc, int freq, const std::string &name);
+
 // Checkpointed pair loop (generalized_correlation.cpp:8-101): the same files
 // as the reference — <base>_ab.npy, and after pair i with i % freq == 0 the
 // whole R (zeros after i) as <base>_<i>.npy, removing <base>_<i-freq>.npy.
@@ -110,6 +113,10 @@ int pair_loop_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k
     validate_xd(xd, platform == GPU_PLATFORM);
     validate_pairs(X, ab, k, n, d);
     const long long P = ab->m();
+    if (platform == GPU_PLATFORM) {
+        gpu_ckpt(X, R, ab, k, n, d, gc, freq, name);
+        return platform;
+    }
     long long start = 0;
     while (start < P) {
         // Next checkpoint index c >= start with c % freq == 0.
@@ -130,6 +137,65 @@ int pair_loop_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k
         start = end;
     }
     return platform;
+}
+
+// The GPU side of the checkpointed loop: X and ab go to the device once, the
+// pairs run as one nsx_ksg_mi_chunked call whose chunks end after each
+// checkpoint index (the list path's tables are built once), and each chunk's
+// callback forms its GC values in R and writes the checkpoint files while
+// the device computes the next chunk.
+struct CkptState {
+    CuArray<float> *R;
+    std::vector<float> *mi;
+    int d;
+    bool gc;
+    long long freq;
+    std::string name;
+    long long prev;
+    std::string error;
+};
+
+int ckpt_chunk_done(void *user, long long end) {
+    CkptState *st = static_cast<CkptState *>(user);
+    try {
+        const long long b = st->prev;
+        std::copy(st->mi->begin() + b, st->mi->begin() + end, st->R->host() + b);
+        if (st->gc) to_gc(st->R->host() + b, end - b, st->d);
+        const long long c = end - 1;  // the chunk ends after pair c
+        if (c % st->freq == 0) {
+            std::remove((st->name + "_" + std::to_string(c - st->freq) + ".npy").c_str());
+            st->R->save(st->name + "_" + std::to_string(c) + ".npy");
+        }
+        st->prev = end;
+        return 0;
+    } catch (const std::exception &e) {
+        st->error = e.what();
+        return 1;
+    }
+}
+
+void gpu_ckpt(CuArray<float> *X, CuArray<float> *R, CuArray<int> *ab, int k, int n, int d,
+              bool gc, int freq, const std::string &name) {
+    const long long P = ab->m(), M = X->m(), ld = X->n();
+    std::vector<long long> ends;
+    for (long long start = 0; start < P;) {
+        const long long c = ((start + freq - 1) / freq) * freq;
+        const long long end = std::min(P, c + 1);
+        ends.push_back(end);
+        start = end;
+    }
+    DeviceBuffer dX((size_t)(M * ld) * sizeof(float));
+    DeviceBuffer dab((size_t)P * 2 * sizeof(int));
+    DeviceBuffer dmi((size_t)P * sizeof(float));
+    check(nsx_memcpy_h2d(dX.ptr, X->host(), (size_t)(M * ld) * sizeof(float)), "upload X");
+    check(nsx_memcpy_h2d(dab.ptr, ab->host(), (size_t)P * 2 * sizeof(int)), "upload ab");
+    std::vector<float> mi((size_t)P);
+    CkptState st{R, &mi, d, gc, freq, name, 0, std::string()};
+    const int rc = nsx_ksg_mi_chunked(dX.as<float>(), ld, n, d, n, dab.as<int>(), P, k,
+                                      dmi.as<float>(), mi.data(), ends.data(), (int)ends.size(),
+                                      ckpt_chunk_done, &st, nullptr);
+    if (!st.error.empty()) throw std::runtime_error("checkpoint: " + st.error);
+    check(rc, "KSG on the GPU");
 }
 
 // One pair from two flat arrays (Xa, Xb hold d*n values at indices i + di*n).

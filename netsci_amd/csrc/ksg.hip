@@ -894,7 +894,11 @@ struct PhaseMarks {
 // Enqueues every window's builds, walks, tie resolution and psi sums (no
 // host synchronisation; capturable once list_prepare has run).
 int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, const float *d_psi,
-                 float *mi_out, int *counts_out, hipStream_t stream, PhaseMarks *pm) {
+                 float *mi_out, int *counts_out, hipStream_t stream, PhaseMarks *pm,
+                 long long q_begin = 0, long long q_end = -1, bool builds = true) {
+    // q_begin / q_end: only the pairs [q_begin, q_end) of the list (a single
+    // block's layout); builds = false skips the table builds (done earlier).
+    if (q_end < 0) q_end = pb.P;
     const size_t lds_cap = 160 * 1024;
     unsigned long long *stats = (unsigned long long *)(L.flags + 8);
     unsigned *tie_count = L.flags + 16;
@@ -917,7 +921,13 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
             1, std::min<long long>(L.chunk, (1LL << 32) / ((long long)n * (long long)sizeof(unsigned))));
         const unsigned wtie_cap = (unsigned)std::min<long long>(
             L.tie_cap, std::max<long long>(1024, wchunk * (long long)n / 32));
-        for (const ListOp &op : L.ops) {
+        for (const ListOp &op0 : L.ops) {
+            ListOp op = op0;
+            if (!op.build && L.nb == 1) {
+                op.p0 = q_begin;
+                op.pc = q_end - q_begin;
+            }
+            if (op.build && !builds) continue;
             if (op.build) {
                 const int first = op.block * L.bn;
                 const int count = std::min(L.bn, L.U - first);
@@ -937,7 +947,8 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 ba.n = n;
                 ba.rs = rs;
                 ba.node_of = L.node_of + first;
-                const int rows_per_pass = ksgw::kBuildWaves / ksgw::build_row_waves(width);
+                const int block_waves = ksgw::build_block_waves(width);
+                const int rows_per_pass = block_waves / ksgw::build_row_waves(width);
                 ba.groups = (int)std::max<long long>(
                     1, std::min<long long>((n + rows_per_pass - 1) / rows_per_pass,
                                            (8LL * cu_count() + count - 1) / std::max(count, 1)));
@@ -945,7 +956,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 ba.rank = L.rank + (long long)base * slot_entries;
                 ba.abort = abort_flag;
                 hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)count * ba.groups)),
-                                   dim3(ksgw::kWave * ksgw::kBuildWaves),
+                                   dim3(ksgw::kWave * block_waves),
                                    ksgw::build_lds_bytes(pb.d, n, rs), stream, ba);
                 NSX_HIP(hipGetLastError(), "list build launch");
                 if (b1) (void)hipEventRecord(b1, stream);
@@ -1104,6 +1115,112 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     g_last_blocks = L.nb;
     g_last_slots = L.S;
     g_last_node_builds = L.node_builds * pb.R;
+    return kOk;
+}
+
+// ---- chunked pair loop (checkpointing callers) ----------------------------------
+
+// All pairs in chunks [ends[c-1], ends[c]): on the list path the tables are
+// built once and every chunk is a range of the same walk layout; otherwise
+// (all-frames method, blocked tables) each chunk is an eager call. After
+// chunk c its MI values are copied to h_mi_out on a second stream and
+// on_chunk(user, ends[c]) runs while the device works on chunks c + 1 and
+// c + 2.
+int run_eager(const Problem &pb, float *mi_out, int *counts_out, hipStream_t stream);
+
+int run_chunked(const Problem &pb, float *mi_out, float *h_mi_out, const long long *ends,
+                int nends, int (*on_chunk)(void *, long long), void *user, hipStream_t stream) {
+    int rc = ensure_device();
+    if (rc) return rc;
+    std::vector<float> psi(pb.nmax + 1);
+    psi_table(psi.data(), pb.nmax);
+    float *d_psi = (float *)workspace(kWsPsi, psi.size() * sizeof(float));
+    if (!d_psi) return kHipError;
+    NSX_HIP(hipMemcpyAsync(d_psi, psi.data(), psi.size() * sizeof(float), hipMemcpyHostToDevice,
+                           stream),
+            "psi upload");
+    bool lists = false;
+    ListLayout L;
+    WalkShape ws;
+    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    const bool fits = pb.k > ksg::kMaxK ? ksgw::bigk_lds_bytes(pb.nmax) <= 160 * 1024
+                                        : rec_bytes + 256 <= 160 * 1024;
+    if ((g_method != 1 || pb.k > ksg::kMaxK) && pb.nmax <= ksgw::kMaxFrames && fits) {
+        rc = walk_shape(pb, ws);
+        if (rc) return rc;
+        bool ok = false;
+        rc = list_prepare(pb, L, stream, &ok, ListMem{});
+        if (rc) return rc;
+        lists = ok && L.nb == 1;
+    }
+    hipStream_t cs = nullptr;
+    NSX_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "copy stream");
+    struct Cleanup {
+        hipStream_t s;
+        std::vector<hipEvent_t> ev;
+        ~Cleanup() {
+            if (s) {
+                (void)hipStreamSynchronize(s);
+                (void)hipStreamDestroy(s);
+            }
+            for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        }
+    } clean{cs, {}};
+    clean.ev.assign(nends, nullptr);
+    for (int c = 0; c < nends; ++c)
+        NSX_HIP(hipEventCreateWithFlags(&clean.ev[c], hipEventDisableTiming), "chunk event");
+    // Hands chunk c's values to the host and calls back.
+    auto finish = [&](int c) -> int {
+        const long long b = c ? ends[c - 1] : 0, e = ends[c];
+        NSX_HIP(hipStreamWaitEvent(cs, clean.ev[c], 0), "chunk wait");
+        NSX_HIP(hipMemcpyAsync(h_mi_out + b, mi_out + b, (size_t)(e - b) * sizeof(float),
+                               hipMemcpyDeviceToHost, cs),
+                "chunk copy");
+        NSX_HIP(hipStreamSynchronize(cs), "chunk copy sync");
+        if (on_chunk && on_chunk(user, e) != 0) {
+            (void)hipStreamSynchronize(stream);
+            set_error("ksg: stopped by the chunk callback");
+            return kInternal;
+        }
+        return kOk;
+    };
+    long long ties = 0;
+    for (int c = 0; c < nends; ++c) {
+        const long long b = c ? ends[c - 1] : 0, e = ends[c];
+        if (lists) {
+            rc = list_enqueue(pb, L, ws, d_psi, mi_out, nullptr, stream, nullptr, b, e, c == 0);
+        } else {
+            Problem sub = pb;
+            sub.ab = pb.ab + 2 * b;
+            sub.P = e - b;
+            rc = run_eager(sub, mi_out + b, nullptr, stream);
+            ties += g_last_ties;
+        }
+        if (rc) return rc;
+        NSX_HIP(hipEventRecord(clean.ev[c], stream), "chunk record");
+        // Two chunks stay queued behind the one being handed over, so the
+        // device keeps working while the callback runs.
+        if (c >= 2) {
+            rc = finish(c - 2);
+            if (rc) return rc;
+        }
+    }
+    for (int c = std::max(0, nends - 2); c < nends; ++c) {
+        rc = finish(c);
+        if (rc) return rc;
+    }
+    if (lists) {
+        unsigned long long h_stats[2] = {0, 0};
+        NSX_HIP(hipMemcpyAsync(h_stats, L.flags + 8, sizeof(h_stats), hipMemcpyDeviceToHost,
+                               stream),
+                "walk stats readback");
+        NSX_HIP(hipStreamSynchronize(stream), "walk sync");
+        g_last_method = 2;
+        g_last_entries = (long long)h_stats[1];
+        g_last_ties = (long long)h_stats[0];
+    } else {
+        g_last_ties = ties;
+    }
     return kOk;
 }
 
@@ -1336,6 +1453,27 @@ int nsx_ksg_mi_noalloc(const float *d_X, long long ld, long long dstride, int d,
     if (rc) return rc;
     if (P == 0) return kOk;
     return run_noalloc(pb, d_mi_out, (hipStream_t)stream);
+}
+
+int nsx_ksg_mi_chunked(const float *d_X, long long ld, long long dstride, int d, int n,
+                       const int *d_ab, long long P, int k, float *d_mi_out, float *h_mi_out,
+                       const long long *h_ends, int nends, nsx_chunk_fn on_chunk, void *user,
+                       void *stream) {
+    TraceRange trace("nsx_ksg_mi_chunked");
+    clear_error();
+    NSX_REQUIRE(d_mi_out && h_mi_out && h_ends && nends >= 1,
+                "nsx_ksg_mi_chunked: invalid arguments");
+    for (int c = 0; c < nends; ++c)
+        NSX_REQUIRE(h_ends[c] > (c ? h_ends[c - 1] : 0) && h_ends[c] <= P,
+                    "nsx_ksg_mi_chunked: chunk ends must increase strictly up to P");
+    NSX_REQUIRE(h_ends[nends - 1] == P, "nsx_ksg_mi_chunked: the last chunk must end at P");
+    const int off = 0;
+    Problem pb;
+    int rc = make_problem(pb, d_X, ld, dstride, d, &off, &n, 1, d_ab, P, k);
+    if (rc) return rc;
+    g_last_method = 0;
+    return run_chunked(pb, d_mi_out, h_mi_out, h_ends, nends, on_chunk, user,
+                       (hipStream_t)stream);
 }
 
 int nsx_ksg_last_tie_frames(long long *count) {

@@ -136,9 +136,18 @@ using BuildFn = void (*)(BuildArgs);
 using WalkFn = void (*)(WalkArgs);
 using TieFn = void (*)(WalkArgs);
 
+// Sort elements per wave-lane for n frames: 8, 16 or 32 for one wave per
+// row; 64, 96 and 128 mean 2, 3 and 4 waves per row (32 each).
+inline int build_width(int n) {
+    return n <= 512 ? 8 : n <= 1024 ? 16 : n <= 2048 ? 32 : n <= 4096 ? 64 : n <= 6144 ? 96 : 128;
+}
+inline int build_row_waves(int width) { return width <= 32 ? 1 : width / 32; }
+// Waves per build workgroup: 4 (4, 2 or 1 rows), and 6 for rows of 3 waves.
+constexpr int build_block_waves(int width) { return width == 96 ? 6 : kBuildWaves; }
+
 // LDS bytes of the two kernels.
 inline size_t build_lds_bytes(int d, int n, long long rs) {
-    if (n > 2048) return (size_t)kBuildWaves * 2048 * 4;  // the wide kernel's exchange rows
+    if (n > 2048) return (size_t)build_block_waves(build_width(n)) * 2048 * 4;  // exchange rows
     return (((size_t)d * n * 4 + 15) & ~(size_t)15) + (size_t)kBuildWaves * rs * 2;
 }
 inline size_t walk_lds_bytes(int d, int n, int nbuf) {
@@ -181,12 +190,6 @@ inline int heap_id_bits(int n) {
 }
 inline int heap_code_shift(int n) { return heap_id_bits(n) > 5 ? heap_id_bits(n) - 5 : 0; }
 
-// Sort elements per wave-lane for n frames: 8, 16 or 32 for one wave per
-// row; 64 and 128 mean 2 and 4 waves per row (32 each).
-inline int build_width(int n) {
-    return n <= 512 ? 8 : n <= 1024 ? 16 : n <= 2048 ? 32 : n <= 4096 ? 64 : 128;
-}
-inline int build_row_waves(int width) { return width <= 32 ? 1 : width / 32; }
 
 // Kernel instances, one translation unit per coordinate dimension.
 BuildFn build_d1(int width);
@@ -498,32 +501,38 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
 // (2048 32-bit values each, element e = lane * 32 + r of wave wl). The LDS
 // copy of a wave's block is stored as [r][lane], so both the writes and the
 // mirrored reads cover 64 consecutive words.
+// A partner wave pw >= W is the all-padding block of a row of W = 3 waves
+// (sorted as 4): an exchange with it leaves this wave's values as they are.
 __device__ __forceinline__ void wave_exchange(unsigned (&v)[32], int lane, int wl, int pw,
-                                              bool mirror, unsigned *xr) {
+                                              bool mirror, unsigned *xr, int W) {
 #pragma unroll
     for (int r = 0; r < 32; ++r) xr[wl * 2048 + r * kWave + lane] = v[r];
     __syncthreads();
-    const bool lower = wl < pw;
+    if (pw < W) {
+        const bool lower = wl < pw;
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-        const unsigned o = mirror ? xr[pw * 2048 + (31 - r) * kWave + (kWave - 1 - lane)]
-                                  : xr[pw * 2048 + r * kWave + lane];
-        v[r] = lower ? min(v[r], o) : max(v[r], o);
+        for (int r = 0; r < 32; ++r) {
+            const unsigned o = mirror ? xr[pw * 2048 + (31 - r) * kWave + (kWave - 1 - lane)]
+                                      : xr[pw * 2048 + r * kWave + lane];
+            v[r] = lower ? min(v[r], o) : max(v[r], o);
+        }
     }
     __syncthreads();
 }
 
-// Merges of sizes 4096 and (W = 4) 8192 after each wave sorted its block:
-// the mirror stage pairs local e of one wave with local 2047 - e of its
-// partner, the stride-2048 stage pairs equal positions, and the strides
-// 1024 .. 1 stay within the wave.
+// Merges of sizes 4096 and (W = 3, 4) 8192 after each wave sorted its
+// block: the mirror stage pairs local e of one wave with local 2047 - e of
+// its partner, the stride-2048 stage pairs equal positions, and the strides
+// 1024 .. 1 stay within the wave. A row of 3 waves is sorted as 4 whose
+// fourth block is all padding (+inf): it stays +inf through every stage,
+// so the exchanges with it change nothing and are skipped.
 template <int W>
 __device__ __forceinline__ void wide_merges32(unsigned (&v)[32], int lane, int wl, unsigned *xr) {
-    wave_exchange(v, lane, wl, wl ^ 1, true, xr);
+    wave_exchange(v, lane, wl, wl ^ 1, true, xr, W);
     asc_stages<32, 1024>(v, lane);
-    if constexpr (W == 4) {
-        wave_exchange(v, lane, wl, 3 - wl, true, xr);
-        wave_exchange(v, lane, wl, wl ^ 1, false, xr);
+    if constexpr (W >= 3) {
+        wave_exchange(v, lane, wl, 3 - wl, true, xr, W);
+        wave_exchange(v, lane, wl, wl ^ 1, false, xr, W);
         asc_stages<32, 1024>(v, lane);
     }
 }
@@ -538,15 +547,18 @@ __device__ __forceinline__ void wide_merges32(unsigned (&v)[32], int lane, int w
 // need their exact key bits (ek): odd-even transposition on (ek, j) inside
 // those runs, across the row's waves, restores exact (key, j) order, and
 // the upper-bound ranks come from the ends of runs of equal (t, ek). A
-// workgroup of kBuildWaves waves holds kBuildWaves / W rows of one node;
-// coordinates are read from global memory (the node's row stays in cache).
+// workgroup of build_block_waves(32 W) waves holds that many / W rows of one
+// node; coordinates are read from global memory (the node's row stays in
+// cache).
 template <int D, int W>
-__global__ __launch_bounds__(kWave *kBuildWaves, 2) void list_build_wide_kernel(BuildArgs a) {
+__global__ __launch_bounds__(kWave *build_block_waves(32 * W), 2) void list_build_wide_kernel(
+    BuildArgs a) {
+    constexpr int kBW = build_block_waves(32 * W);
     extern __shared__ __attribute__((aligned(16))) unsigned xch[];
-    __shared__ unsigned edge_u[2][kBuildWaves], edge_k[2][kBuildWaves];  // [0] first, [1] last
-    __shared__ unsigned wave_min[kBuildWaves];
+    __shared__ unsigned edge_u[2][kBW], edge_k[2][kBW];  // [0] first, [1] last
+    __shared__ unsigned wave_min[kBW];
     __shared__ int any_swap;
-    constexpr int kRows = kBuildWaves / W;
+    constexpr int kRows = kBW / W;
     constexpr unsigned kPad = 0xffffffffu;  // above every packed finite key
     const int n = a.n;
     const int slot = blockIdx.x / a.groups;
@@ -1641,6 +1653,7 @@ BuildFn build_lookup(int width) {
         case 16: return list_build_kernel<D, 16>;
         case 32: return list_build_kernel<D, 32>;
         case 64: return list_build_wide_kernel<D, 2>;
+        case 96: return list_build_wide_kernel<D, 3>;
         case 128: return list_build_wide_kernel<D, 4>;
         default: return nullptr;
     }

@@ -2,13 +2,14 @@
 generalized_correlation.cpp:8-101, the values equal to the oracle's pair
 loop (the reference CPU path restated) and to the CPU platform's run of the
 same call."""
+import ctypes
 import os
 
 import numpy as np
 import pytest
 
 import oracle
-from netsci_amd import cuarray, netcalc, synth
+from netsci_amd import abi, cuarray, netcalc, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -48,3 +49,67 @@ def test_gpu_checkpoints_equal_cpu(tmp_path):
     np.testing.assert_array_equal(out["gpu"][2].ravel(), want)
     assert out["gpu"][1] == out["cpu"][1] == ["ck_20.npy", "ck_ab.npy"]
     np.testing.assert_array_equal(out["gpu"][2], out["cpu"][2])
+
+
+_CHUNK_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong)
+
+
+def _chunked(X, ab, k, n, d, ends, stop_at=None):
+    """nsx_ksg_mi_chunked through the C ABI; returns the final values and a
+    copy of the host output taken at every callback."""
+    X = np.ascontiguousarray(X, np.float32)
+    P = len(ab)
+    dX = abi.DeviceArray.from_numpy(X)
+    dab = abi.DeviceArray.from_numpy(np.ascontiguousarray(ab, np.int32))
+    dmi = abi.DeviceArray((P,), np.float32)
+    host = np.full(P, np.nan, np.float32)
+    e = np.ascontiguousarray(ends, np.int64)
+    seen = []
+
+    def cb(user, end):
+        seen.append((int(end), host.copy()))
+        return 1 if stop_at is not None and end >= stop_at else 0
+
+    fn = _CHUNK_FN(cb)
+    rc = abi.lib().nsx_ksg_mi_chunked(
+        dX.ptr, X.shape[1], n, d, n, dab.ptr, P, k, dmi.ptr, host.ctypes.data,
+        e.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(e), fn, None, None)
+    return rc, host, seen
+
+
+@pytest.mark.parametrize("method,budget_slots", [(abi.METHOD_AUTO, 0), (abi.METHOD_AUTO, 4),
+                                                 (abi.METHOD_ALL_FRAMES, 0)])
+def test_chunked_pair_loop_values_at_every_chunk(method, budget_slots):
+    """Every callback sees exactly the finished chunks' values (equal to
+    nsx_ksg_mi's and the oracle's) and nothing after them; on the list path
+    the tables are built once (blocked tables and the all-frames method run
+    each chunk as its own call)."""
+    print(f"chunked method={method} budget={budget_slots}", flush=True)
+    N, n, d, k = 12, 300, 3, 6
+    X, _ = synth.calpha_trajectory(N=N, n=n, seed=31)
+    ab = synth.all_pairs(N)  # 66 pairs
+    ends = [1, 11, 21, 31, 41, 51, 61, 66]
+    want = oracle.pairs(X, ab, k, n, d, gc=False)
+    rows = (n + 7) // 8 * 8
+    abi.set_method(method)
+    abi.set_table_budget(budget_slots * 2 * rows * rows * 2)
+    try:
+        rc, host, seen = _chunked(X, ab, k, n, d, ends)
+    finally:
+        abi.set_method(abi.METHOD_AUTO)
+        abi.set_table_budget(0)
+    assert rc == 0, abi.lib().nsx_last_error()
+    np.testing.assert_array_equal(host, want)
+    assert [s for s, _ in seen] == ends
+    for end, snap in seen:
+        np.testing.assert_array_equal(snap[:end], want[:end])
+        assert np.isnan(snap[end:]).all()
+
+
+def test_chunked_callback_stops_the_call():
+    N, n, d, k = 6, 200, 3, 4
+    X, _ = synth.calpha_trajectory(N=N, n=n, seed=32)
+    ab = synth.all_pairs(N)
+    rc, _, seen = _chunked(X, ab, k, n, d, [5, 10, 15], stop_at=10)
+    assert rc != 0 and b"callback" in abi.lib().nsx_last_error()
+    assert [s for s, _ in seen] == [5, 10]
