@@ -366,6 +366,13 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
                    (long long)wib * a.rs;
     constexpr unsigned kPad = 0xffffffffu;  // above every packed finite key
     for (int i = g * kBuildWaves + wib; i < n; i += a.groups * kBuildWaves) {
+        // The lane index is re-derived every row from a value the compiler
+        // cannot see through, so per-lane addresses and bounds are computed
+        // where they are used instead of being hoisted out of the loop and
+        // held in registers across it.
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & (kWave - 1);
         float q[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = lds[d * n + i];
@@ -551,7 +558,7 @@ __device__ __forceinline__ void wide_merges32(unsigned (&v)[32], int lane, int w
 // node; coordinates are read from global memory (the node's row stays in
 // cache).
 template <int D, int W>
-__global__ __launch_bounds__(kWave *build_block_waves(32 * W), 2) void list_build_wide_kernel(
+__global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_build_wide_kernel(
     BuildArgs a) {
     constexpr int kBW = build_block_waves(32 * W);
     extern __shared__ __attribute__((aligned(16))) unsigned xch[];
@@ -572,6 +579,13 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 2) void list_buil
     unsigned *xr = xch + rg * (W * 2048);
     uint16_t *st = reinterpret_cast<uint16_t *>(xr);  // rank staging after the sort
     for (int base = g * kRows; base < n; base += a.groups * kRows) {
+        // The lane index is re-derived every row from a value the compiler
+        // cannot see through, so per-lane addresses and bounds are computed
+        // where they are used instead of being hoisted out of the loop and
+        // held in registers across it.
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & (kWave - 1);
         const int i = base + rg;
         const bool row = i < n;
         float q[D];
