@@ -326,11 +326,11 @@ ksgw::WalkFn walk_for(int d, int k) {
     }
 }
 
-ksgw::TieFn tie_for(int d, int k, bool overflow) {
+ksgw::TieFn tie_for(int d, int k, int kind) {
     switch (d) {
-        case 1: return ksgw::tie_d1(k, overflow);
-        case 2: return ksgw::tie_d2(k, overflow);
-        case 3: return ksgw::tie_d3(k, overflow);
+        case 1: return ksgw::tie_d1(k, kind);
+        case 2: return ksgw::tie_d2(k, kind);
+        case 3: return ksgw::tie_d3(k, kind);
         default: return nullptr;
     }
 }
@@ -563,7 +563,7 @@ struct ListLayout {
 // Fixed launch shape of the walk for this problem.
 struct WalkShape {
     ksgw::WalkFn walk = nullptr;
-    ksgw::TieFn tiek = nullptr, tieo = nullptr;
+    ksgw::TieFn ties = nullptr, tiek = nullptr, tieo = nullptr;
     int wpb = 16;
     long long grid_max = 0;
 };
@@ -589,9 +589,10 @@ int walk_shape(const Problem &pb, WalkShape &ws) {
         return kOk;
     }
     ws.walk = walk_for(pb.d, pb.k);
-    ws.tiek = tie_for(pb.d, pb.k, false);
-    ws.tieo = tie_for(pb.d, pb.k, true);
-    NSX_REQUIRE(ws.walk && ws.tiek && ws.tieo, "ksg: no list-path kernel for (d, k)");
+    ws.ties = tie_for(pb.d, pb.k, ksgw::kTieSelect);
+    ws.tiek = tie_for(pb.d, pb.k, ksgw::kTieExact);
+    ws.tieo = tie_for(pb.d, pb.k, ksgw::kTieOverflow);
+    NSX_REQUIRE(ws.walk && ws.ties && ws.tiek && ws.tieo, "ksg: no list-path kernel for (d, k)");
     const size_t lds_cap = 160 * 1024;
     const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
     ws.wpb = std::min(g_walk_wpb, ksgw::walk_max_waves(pb.k));
@@ -869,7 +870,7 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok,
         1, std::min<long long>(pb.P, (1LL << 32) / ((long long)pb.nmax * (long long)sizeof(unsigned))));
     L.tie_cap = (unsigned)std::max<long long>(1024, L.chunk * (long long)pb.nmax / 32);
     const size_t cnt_words = (size_t)L.chunk * pb.nmax + pb.nmax;
-    L.cnt = (unsigned *)mem.get(kWsListCnt, (cnt_words + L.tie_cap) * sizeof(unsigned));
+    L.cnt = (unsigned *)mem.get(kWsListCnt, (cnt_words + 2 * (size_t)L.tie_cap) * sizeof(unsigned));
     if (!L.cnt) return kHipError;
     L.tie_list = L.cnt + cnt_words;
     *ok = true;
@@ -954,6 +955,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                                            (8LL * cu_count() + count - 1) / std::max(count, 1)));
                 ba.ord = L.ord + (long long)base * slot_entries;
                 ba.rank = L.rank + (long long)base * slot_entries;
+                ba.span = span;
                 ba.abort = abort_flag;
                 hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)count * ba.groups)),
                                    dim3(ksgw::kWave * block_waves),
@@ -986,6 +988,8 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 wa.tie_list = L.tie_list;
                 wa.tie_count = tie_count;
                 wa.tie_cap = wtie_cap;
+                wa.exact_list = L.tie_list + L.tie_cap;
+                wa.exact_count = tie_count + 1;
                 wa.nbuf = nbuf;
                 wa.batch = g_walk_batch;
                 wa.checkpoint = 24;
@@ -994,7 +998,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 wa.code_shift = ksgw::heap_code_shift(n);
                 wa.abort = abort_flag;
                 if (!L.tables) wa.rank = nullptr;
-                int rc = fill_u32(tie_count, 0u, 1, stream);
+                int rc = fill_u32(tie_count, 0u, 2, stream);
                 if (rc) return rc;
                 if (e0) (void)hipEventRecord(e0, stream);
                 if (pb.k > ksg::kMaxK) {
@@ -1013,7 +1017,12 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                                    stream, wa);
                 NSX_HIP(hipGetLastError(), "walk launch");
                 if (e1) (void)hipEventRecord(e1, stream);
-                // Up to 4 waves per workgroup, each with its LDS row and lists.
+                // Every deferred query (no LDS: 8 workgroups of 4 waves per
+                // CU), then the exact ties among them with up to 4 waves per
+                // workgroup, each with its LDS row and lists.
+                hipLaunchKernelGGL(ws.ties, dim3((unsigned)tie_waves), dim3(ksgw::kWave * 4), 0,
+                                   stream, wa);
+                NSX_HIP(hipGetLastError(), "tie select launch");
                 const size_t tie_wave_bytes = ksgw::tie_wave_lds_bytes(n);
                 const int tie_wpb =
                     (int)std::max<size_t>(1, std::min<size_t>(4, lds_cap / tie_wave_bytes));

@@ -82,6 +82,8 @@ struct BuildArgs {
     int groups;         // workgroups per node
     uint16_t *ord;      // [slot][i][rs] frames sorted by (D, j)
     uint16_t *rank;     // [slot][i][rs] upper-bound ranks, indexed by frame
+    const float *span;  // per node: max over dimensions of (max - min) of the
+                        // coordinate, a bound of every marginal distance
     const unsigned *abort;  // nullable: return at once when *abort != 0 (a
                             // non-finite coordinate was found beforehand)
 };
@@ -106,6 +108,8 @@ struct WalkArgs {
     unsigned *tie_count;        // entries appended (may exceed tie_cap; the rest are
                                 // marked kTieMark in cnt)
     unsigned tie_cap;
+    unsigned *exact_list;       // tie-list entries whose k-th and (k+1)-th joint
+    unsigned *exact_count;      // distances are equal (tie_cap capacity)
     int nbuf;                   // pair buffers per workgroup (LDS ring)
     int batch;                  // lanes that wait before a refill round
     int checkpoint;             // loop passes between workgroup checkpoints
@@ -198,10 +202,13 @@ BuildFn build_d3(int width);
 WalkFn walk_d1(int k);
 WalkFn walk_d2(int k);
 WalkFn walk_d3(int k);
-// overflow: tie_overflow_kernel instead of tie_kernel.
-TieFn tie_d1(int k, bool overflow);
-TieFn tie_d2(int k, bool overflow);
-TieFn tie_d3(int k, bool overflow);
+// Tie-resolution kernels by kind: kTieSelect (every deferred query, no
+// LDS), kTieExact (the exact ties among them, row in LDS), kTieOverflow
+// (queries the tie list had no room for).
+enum TieKind : int { kTieSelect = 0, kTieExact = 1, kTieOverflow = 2 };
+TieFn tie_d1(int k, int kind);
+TieFn tie_d2(int k, int kind);
+TieFn tie_d3(int k, int kind);
 // The one-wave-per-query kernel for k > ksg::kMaxK (any k).
 WalkFn bigk_d1();
 WalkFn bigk_d2();
@@ -229,15 +236,32 @@ __device__ __forceinline__ unsigned xor_lane(unsigned v) {
     }
 }
 
+// Sort keys of the table builds: a marginal distance as a code of CB bits
+// relative to the node's span (no marginal distance of the node exceeds
+// it), 2^(27 - CB) float-bit values per step, so the codes cover the 2^27
+// float-bit values (16 binades) below the span; smaller keys share code 0.
+// The code is monotone in the key. Its largest value is 2^CB - 2, so no
+// packed (code << (32 - CB)) | j equals the padding 0xffffffff.
+template <int CB>
+__device__ __forceinline__ unsigned build_code_base(float span) {
+    constexpr unsigned kMaxOff = (((1u << CB) - 2u) << (27 - CB)) | ((1u << (27 - CB)) - 1u);
+    const unsigned top = __float_as_uint(span);
+    return top > kMaxOff ? top - kMaxOff : 0u;
+}
+template <int CB>
+__device__ __forceinline__ unsigned build_code(float key, unsigned base) {
+    return __builtin_elementwise_sub_sat(__float_as_uint(key), base) >> (27 - CB);
+}
+
 // ---- rows of n <= 2048 frames: one wave per row, 32-bit sort keys ----------
 //
-// A row's elements are packed as (the key's float bits >> 10) << 11 | j: the
-// truncated key keeps the order of the non-negative keys (equal truncations
-// only for keys within 2^-13 relative of each other) and j < 2048 fits the
-// low bits. These 32-bit values are sorted with every merge ascending (the
+// A row's elements are packed as build_code<21>(key) << 11 | j: equal codes
+// only for keys within 2^-17 relative of each other (or both below the
+// node's span by more than 16 binades), and j < 2048 fits the low bits.
+// These 32-bit values are sorted with every merge ascending (the
 // first stage of a merge compares mirrored positions), so a compare-exchange
 // is one v_min_u32 and one v_max_u32. The exact (key, j) order inside runs of
-// equal truncated keys is then restored by odd-even transposition on the exact
+// equal codes is then restored by odd-even transposition on the exact
 // keys, repeated until no pair swaps.
 
 // Partner value of the lane (lane ^ M), for any M < 64.
@@ -356,6 +380,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = threadIdx.x / kWave;
     if (a.abort && *a.abort) return;
+    const unsigned cbase = build_code_base<21>(a.span[a.node_of[slot]]);
     for (int t = threadIdx.x; t < D * n; t += blockDim.x) {
         const int d = t / n, f = t - d * n;
         lds[t] = x[d * a.dstride + f];
@@ -387,7 +412,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
                 float c[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) c[d] = lds[d * n + e];
-                v[r] = ((__float_as_uint(marg<D>(q, c)) >> 10) << 11) | (unsigned)e;
+                v[r] = (build_code<21>(marg<D>(q, c), cbase) << 11) | (unsigned)e;
             } else {
                 v[r] = kPad;
             }
@@ -395,8 +420,8 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
         mirror_merges<R, 2>(v, lane);
 
         // Exact order: the elements as (exact key bits << 32 | j), then
-        // odd-even transposition until no pair swaps. Pairs whose truncated
-        // keys differ are already in exact order, so only equal-truncation
+        // odd-even transposition until no pair swaps. Pairs whose codes
+        // differ are already in exact order, so only equal-code
         // runs (typically of 2 or 3) move; the loop ends after as many passes
         // as the longest run.
         unsigned long long ex[R];
@@ -545,12 +570,13 @@ __device__ __forceinline__ void wide_merges32(unsigned (&v)[32], int lane, int w
 }
 
 // Rows of 2048 < n <= 2048 W frames: W waves sort one row together (each a
-// 2048-element block, 32 per lane) as 32-bit values (key bits >> 12) << 13
-// | j: every in-wave compare-exchange is a v_min_u32 / v_max_u32 with DPP
-// or permlane partners, as in list_build_kernel, and the cross-wave stages
-// go through LDS. The truncated key t = value >> 13 keeps the order of the
-// non-negative keys (equal truncations only within 2^-11 relative) and
-// j < 8192 fits the low bits. Only elements that share t with a neighbour
+// 2048-element block, 32 per lane) as 32-bit values build_code<19>(key) <<
+// 13 | j: every in-wave compare-exchange is a v_min_u32 / v_max_u32 with
+// DPP or permlane partners, as in list_build_kernel, and the cross-wave
+// stages go through LDS. The code t = value >> 13 is monotone in the key
+// (equal codes only within 2^-15 relative, or both more than 16 binades
+// below the node's span) and j < 8192 fits the low bits. Only elements that
+// share t with a neighbour
 // need their exact key bits (ek): odd-even transposition on (ek, j) inside
 // those runs, across the row's waves, restores exact (key, j) order, and
 // the upper-bound ranks come from the ends of runs of equal (t, ek). A
@@ -576,6 +602,7 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
     const int rg = wib / W, wl = wib - rg * W;  // row in the workgroup, block in the row
     const int ebase = wl * 2048;
     if (a.abort && *a.abort) return;
+    const unsigned cbase = build_code_base<19>(a.span[a.node_of[slot]]);
     unsigned *xr = xch + rg * (W * 2048);
     uint16_t *st = reinterpret_cast<uint16_t *>(xr);  // rank staging after the sort
     for (int base = g * kRows; base < n; base += a.groups * kRows) {
@@ -601,7 +628,7 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
                 float c[D];
 #pragma unroll
                 for (int d = 0; d < D; ++d) c[d] = x[d * a.dstride + e];
-                u[r] = ((__float_as_uint(marg<D>(q, c)) >> 12) << 13) | (unsigned)e;
+                u[r] = (build_code<19>(marg<D>(q, c), cbase) << 13) | (unsigned)e;
             } else {
                 u[r] = kPad;
             }
@@ -622,7 +649,7 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
         if (lane == kWave - 1) un = wl + 1 < W ? edge_u[0][wib + 1] : kPad;
         if (lane == 0) up = wl > 0 ? edge_u[1][wib - 1] : kPad;
         __syncthreads();
-        // Exact key bits of the elements in runs of equal truncation.
+        // Exact key bits of the elements in runs of equal code.
         unsigned ek[32];
 #pragma unroll
         for (int r = 0; r < 32; ++r) {
@@ -640,7 +667,7 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
             if ((r & 7) == 7) asm volatile("" ::: "memory");
         }
         // Odd-even transposition within runs, until no pair of the
-        // workgroup's rows swaps. A pair with different truncations is in
+        // workgroup's rows swaps. A pair with different codes is in
         // order already.
         auto cx = [](unsigned &ux, unsigned &kx, unsigned &uy, unsigned &ky) {
             const bool sw = (ux >> 13) == (uy >> 13) &&
@@ -724,7 +751,7 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
 
         // Upper-bound ranks: the first end of a run of equal exact keys at or
         // after each position, across the row's waves. Position e ends a run
-        // when its successor has another truncation or exact key.
+        // when its successor has another code or exact key.
         if (lane == 0) {
             edge_u[0][wib] = u[0];
             edge_k[0][wib] = ek[0];
@@ -1426,8 +1453,112 @@ __device__ void resolve_tie(const WalkArgs &a, long long pl, int i, char *wave_l
     ksg::wave_sync();
 }
 
-// The tie list's queries, one wave each, with tie_wave_lds_bytes(n) of LDS
-// per wave (the row of n records and the partition's lists).
+// The k+1 smallest joint distances of query frame i over the frames j != i,
+// exact (each lane its frames, one med3 per slot, then a butterfly merge:
+// every lane ends with the row's k+1 smallest). When the k-th and (k+1)-th
+// differ, the k nearest are the frames at or below the k-th: eps is their
+// largest marginal distance and the counts come from the rank tables, as in
+// resolve_tie; the function writes them and returns true. An exact tie
+// returns false (its selection needs the whole row, in libstdc++ order).
+template <int D, int K>
+__device__ bool select_nearest(const WalkArgs &a, long long pl, int i, int lane) {
+    const int n = a.n;
+    const long long p = a.p0 + pl;
+    const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
+    const float *xa = a.X + (long long)na * a.ld;
+    const float *xb = a.X + (long long)nb * a.ld;
+    float qa[D], qb[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        qa[d] = xa[d * a.dstride + i];
+        qb[d] = xb[d * a.dstride + i];
+    }
+    float h[K + 1];
+#pragma unroll
+    for (int m = 0; m <= K; ++m) h[m] = __builtin_inff();
+    for (int j = lane; j < n; j += kWave) {
+        float va[D], vb[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            va[d] = xa[d * a.dstride + j];
+            vb[d] = xb[d * a.dstride + j];
+        }
+        const float z = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
+        const float c = j == i ? __builtin_inff() : z;
+#pragma unroll
+        for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+        h[0] = fminf(h[0], c);
+    }
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        float o[K + 1];
+#pragma unroll
+        for (int m = 0; m <= K; ++m) o[m] = __shfl_xor(h[m], off);
+#pragma unroll
+        for (int t = 0; t <= K; ++t) {
+#pragma unroll
+            for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], o[t], h[m]);
+            h[0] = fminf(h[0], o[t]);
+        }
+    }
+    if (!(h[K - 1] < h[K])) return false;
+    float ex = -1.0f, ey = -1.0f;
+    int jx = 0, jy = 0;
+    for (int j = lane; j < n; j += kWave) {
+        float va[D], vb[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            va[d] = xa[d * a.dstride + j];
+            vb[d] = xb[d * a.dstride + j];
+        }
+        const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
+        if (j == i || !(fmaxf(da, db) <= h[K - 1])) continue;
+        jx = da > ex ? j : jx;
+        ex = fmaxf(ex, da);
+        jy = db > ey ? j : jy;
+        ey = fmaxf(ey, db);
+    }
+    // Any frame of the largest marginal distance gives the same rank.
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const float oex = __shfl_xor(ex, off), oey = __shfl_xor(ey, off);
+        const int ojx = __shfl_xor(jx, off), ojy = __shfl_xor(jy, off);
+        jx = oex > ex ? ojx : jx;
+        ex = fmaxf(ex, oex);
+        jy = oey > ey ? ojy : jy;
+        ey = fmaxf(ey, oey);
+    }
+    if (lane == 0) {
+        const long long offA = table_row(a.slot_of[na], i, n, a.rs);
+        const long long offB = table_row(a.slot_of[nb], i, n, a.rs);
+        const unsigned rx = a.rank[table_at(offA, jx)], ry = a.rank[table_at(offB, jy)];
+        a.cnt[pl * n + i] = (rx - 1u) | ((ry - 1u) << 16);
+    }
+    return true;
+}
+
+// Every entry of the tie list, one wave each and no LDS (so the CUs hold
+// many waves): most entries are distance-code collisions, which
+// select_nearest settles; the exact ties go to the exact list for
+// tie_kernel.
+template <int D, int K>
+__global__ __launch_bounds__(256) void tie_select_kernel(WalkArgs a) {
+    if (a.abort && *a.abort) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
+    const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
+    for (long long e = wave; e < count; e += nwaves) {
+        const unsigned v = a.tie_list[e];
+        const long long pl = v / (unsigned)a.n;
+        const bool done = select_nearest<D, K>(a, pl, (int)(v - pl * a.n), lane);
+        if (!done && lane == 0) a.exact_list[atomicAdd(a.exact_count, 1u)] = v;
+    }
+}
+
+// The exact ties of the tie list (kept by tie_select_kernel), one wave each,
+// with tie_wave_lds_bytes(n) of LDS per wave (the row of n records and the
+// partition's lists).
 template <int D, int K>
 __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) char tie_lds[];
@@ -1436,9 +1567,9 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
     const long long wave = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const long long nwaves = (long long)gridDim.x * (blockDim.x / kWave);
     char *wl = tie_lds + (threadIdx.x / kWave) * tie_wave_lds_bytes(a.n);
-    const long long count = min((long long)*a.tie_count, (long long)a.tie_cap);
+    const long long count = (long long)*a.exact_count;
     for (long long e = wave; e < count; e += nwaves) {
-        const unsigned v = a.tie_list[e];
+        const unsigned v = a.exact_list[e];
         const long long pl = v / (unsigned)a.n;
         resolve_tie<D, K>(a, pl, (int)(v - pl * a.n), wl, lane);
     }
@@ -1623,24 +1754,26 @@ __global__ __launch_bounds__(kWave) void bigk_kernel(WalkArgs a) {
 
 template <int D, int K>
 struct WalkInstances {
-    static void fill(WalkFn *t, TieFn *u, TieFn *o) {
+    static void fill(WalkFn *t, TieFn *s, TieFn *u, TieFn *o) {
         t[K] = walk_kernel<D, K>;
+        s[K] = tie_select_kernel<D, K>;
         u[K] = tie_kernel<D, K>;
         o[K] = tie_overflow_kernel<D, K>;
-        WalkInstances<D, K - 1>::fill(t, u, o);
+        WalkInstances<D, K - 1>::fill(t, s, u, o);
     }
 };
 template <int D>
 struct WalkInstances<D, 0> {
-    static void fill(WalkFn *, TieFn *, TieFn *) {}
+    static void fill(WalkFn *, TieFn *, TieFn *, TieFn *) {}
 };
 
 template <int D>
 struct WalkTable {
     WalkFn t[ksg::kMaxK + 1] = {};
+    TieFn s[ksg::kMaxK + 1] = {};
     TieFn u[ksg::kMaxK + 1] = {};
     TieFn o[ksg::kMaxK + 1] = {};
-    WalkTable() { WalkInstances<D, ksg::kMaxK>::fill(t, u, o); }
+    WalkTable() { WalkInstances<D, ksg::kMaxK>::fill(t, s, u, o); }
 };
 
 template <int D>
@@ -1655,9 +1788,10 @@ WalkFn walk_lookup(int k) {
 }
 
 template <int D>
-TieFn tie_lookup(int k, bool overflow) {
+TieFn tie_lookup(int k, int kind) {
     if (k < 1 || k > ksg::kMaxK) return nullptr;
-    return overflow ? walk_table<D>().o[k] : walk_table<D>().u[k];
+    return kind == kTieSelect ? walk_table<D>().s[k]
+                              : (kind == kTieOverflow ? walk_table<D>().o[k] : walk_table<D>().u[k]);
 }
 
 template <int D>

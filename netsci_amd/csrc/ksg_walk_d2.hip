@@ -7,7 +7,7 @@ namespace ksgw {
 
 BuildFn build_d2(int width) { return build_lookup<2>(width); }
 WalkFn walk_d2(int k) { return walk_lookup<2>(k); }
-TieFn tie_d2(int k, bool overflow) { return tie_lookup<2>(k, overflow); }
+TieFn tie_d2(int k, int kind) { return tie_lookup<2>(k, kind); }
 WalkFn bigk_d2() { return bigk_kernel<2>; }
 
 }  // namespace ksgw

@@ -7,7 +7,7 @@ namespace ksgw {
 
 BuildFn build_d3(int width) { return build_lookup<3>(width); }
 WalkFn walk_d3(int k) { return walk_lookup<3>(k); }
-TieFn tie_d3(int k, bool overflow) { return tie_lookup<3>(k, overflow); }
+TieFn tie_d3(int k, int kind) { return tie_lookup<3>(k, kind); }
 WalkFn bigk_d3() { return bigk_kernel<3>; }
 
 }  // namespace ksgw
