@@ -474,7 +474,7 @@ def bench_screened(args, abi, synth):
     recall). Then, on a fresh 4000-node trajectory: the plain pass on the
     sorted-list path over every pair (the fastest plain GPU path, and the
     recall reference; its tables, 400 GB for all nodes, run on the blocked
-    schedule), the all-frames kernel timed on a 200,000-pair sample and
+    schedule; the faster of two calls), the all-frames kernel timed on a 200,000-pair sample and
     extrapolated, and the screened pass (features, MLP, selection, exact KSG
     on candidates plus a 1% audit). The speedup divides the fastest plain
     pass by the screened pass."""
@@ -496,18 +496,24 @@ def bench_screened(args, abi, synth):
     dab = abi.DeviceArray.from_numpy(ab)
     dmi = abi.DeviceArray((P,), np.float32)
     timer = abi.Timer()
-    # Plain pass on the sorted-list path, every pair, one call.
-    log(f"plain pass (sorted lists), {P} pairs")
-    timer.start()
-    abi.ksg_mi(dX, X.shape[1], n, d, n, dab, P, k, dmi)
-    timer.stop()
-    t_plain = timer.ms() * 1e-3
-    plain_method = abi.last_method()[0]
-    plain_phases = abi.last_phase_ms()
-    blocks, slots, builds = abi.last_list_info()
+    # Plain pass on the sorted-list path, every pair, one call, twice: the
+    # first call also grows the library's table buffer to its budget (a
+    # fresh allocation of about 240 GB), the second runs as the screened
+    # pass later does, on the buffer already held.
+    plain_calls = []
+    for call in range(2):
+        log(f"plain pass (sorted lists), {P} pairs, call {call + 1}")
+        timer.start()
+        abi.ksg_mi(dX, X.shape[1], n, d, n, dab, P, k, dmi)
+        timer.stop()
+        plain_calls.append(timer.ms() * 1e-3)
+        plain_method = abi.last_method()[0]
+        plain_phases = abi.last_phase_ms()
+        blocks, slots, builds = abi.last_list_info()
+        log(f"plain {plain_calls[-1]:.2f} s (method {plain_method}, {blocks} node blocks, "
+            f"{builds} node tables built, phases {plain_phases})")
+    t_plain = min(plain_calls)
     plain = abi.gc_from_mi(dmi.numpy(), d)
-    log(f"plain {t_plain:.2f} s (method {plain_method}, {blocks} node blocks, {builds} node "
-        f"tables built, phases {plain_phases})")
     # All-frames kernel on a seeded sample, extrapolated.
     rng = np.random.default_rng(5)
     sample = ab[np.sort(rng.choice(P, size=200000, replace=False))]
@@ -543,7 +549,8 @@ def bench_screened(args, abi, synth):
         "dtype": "f32 (KSG), bf16 MFMA (screener)", "data": "synthetic",
         "config": {"workload": f"all-pairs GC N={N}, d={d}, n={n}, k={k}, {P} pairs",
                    "training": "8 trajectories x 60000 pairs, N=1000, n=5000"},
-        "plain_lists_s": t_plain, "plain_lists_phases_ms": plain_phases,
+        "plain_lists_s": t_plain, "plain_lists_calls_s": plain_calls,
+        "plain_lists_phases_ms": plain_phases,
         "plain_lists_node_blocks": blocks, "plain_lists_node_tables_built": builds,
         "plain_all_frames_s_extrapolated": t_af,
         "plain_all_frames_sample": f"{len(sample)} seeded pairs, equal to the list path: {af_same}",

@@ -42,8 +42,10 @@
 //     they read from is staged in LDS once per step;
 //   * dense graphs: one WAVE per (i, j) tests 64 q at once against a
 //     transposed copy of A (coalesced) and a ballot picks the first hit.
-// A device flag written by the scan selects the kernel, so the host never
-// waits; the other kernel exits at once.
+// A device flag written by the scan selects the kernel: the backtrack
+// launches both variants and the other one exits at once; the steps read
+// the flag of A's lists back once per call and launch only the one that
+// works (L launches that exit at once would cost more than the readback).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -168,24 +170,27 @@ __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ 
 // 14.3 ms; buffer loads with the row offset in a scalar register 17.2 ms.
 // Counters (N = 4000 step): 5.8 M row reads, L1 hit rate ~93%, 2.5 TB/s from
 // HBM, VALU ~55% busy.
+// TJ = 16 (one j per wave) for small N, where the 64-wide tiles would give
+// each CU only a few waves, each with a chain of four lists to walk.
 constexpr int kStepWaves = 16;  // waves per sparse-step workgroup
 
+template <int TJ>
 __global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
     const float *__restrict__ Hp, const long long *__restrict__ rows, const int *__restrict__ lst,
     const unsigned *__restrict__ dense, float *__restrict__ out, int N, int nI, int nJ) {
-    __shared__ float T[64][65];
+    __shared__ float T[64][TJ + 1];
     if (dense[0]) return;
     const int g = blockIdx.x;
     const int s = g >> 3;
     const int iI = (g & 7) + 8 * (s / nJ);
     const int jJ = s % nJ;
     if (iI >= nI) return;
-    const int i0 = iI * 64, j0 = jJ * 64;
+    const int i0 = iI * 64, j0 = jJ * TJ;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int i = i0 + lane;
     const float *col = Hp + (i < N ? i : 0);  // col[k * N] = Hp[k][i]
-    for (int jj = w; jj < 64; jj += kStepWaves) {
+    for (int jj = w; jj < TJ; jj += kStepWaves) {
         const int j = j0 + jj;
         float c = __builtin_inff();
         if (j < N) {
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
     __syncthreads();
     for (int r = w; r < 64; r += kStepWaves) {
         const int ii = i0 + r, j = j0 + lane;
-        if (ii < N && j < N) out[(long long)ii * N + j] = T[r][lane];
+        if (lane < TJ && ii < N && j < N) out[(long long)ii * N + j] = T[r][lane];
     }
 }
 
@@ -580,6 +585,7 @@ struct Lists {
     long long *rows = nullptr;
     int *lst = nullptr;  // groups of 4 entries: 4 node ids, then 4 weights
     unsigned *dense = nullptr;
+    bool host_dense = false;  // *dense, when read back (step lists only)
 };
 
 int build_lists(const float *M, int N, int slot_rows, int slot_cand, bool allow_sparse,
@@ -602,16 +608,26 @@ int build_lists(const float *M, int N, int slot_rows, int slot_cand, bool allow_
     return kOk;
 }
 
-// One Jacobi step out = step(Hp). Both variants are launched; the dense
-// flag of A's neighbour lists lets exactly one of them do the work, so the
-// host never reads the flag back.
+// One Jacobi step out = step(Hp): the sparse or the dense kernel, as the
+// dense flag of A's neighbour lists (read back once per call) says.
 int minplus_step(const float *A, const float *Hp, float *out, int N, const unsigned *negzero,
                  const Lists &la, hipStream_t s) {
-    const int nT = (N + 63) / 64;
-    const int groups = 8 * ((nT + 7) / 8) * nT;
-    hipLaunchKernelGGL(minplus_sparse_kernel, dim3(groups), dim3(kStepWaves * 64), 0, s, Hp,
-                       la.rows, la.lst, la.dense, out, N, nT, nT);
-    NSX_HIP(hipGetLastError(), "sparse min-plus launch");
+    if (!la.host_dense) {
+        const int nI = (N + 63) / 64;
+        // 64-wide j tiles when they give every CU at least four workgroups.
+        if ((long long)nI * nI >= 4LL * cu_count()) {
+            const int groups = 8 * ((nI + 7) / 8) * nI;
+            hipLaunchKernelGGL(minplus_sparse_kernel<64>, dim3(groups), dim3(kStepWaves * 64), 0,
+                               s, Hp, la.rows, la.lst, la.dense, out, N, nI, nI);
+        } else {
+            const int nJ = (N + 15) / 16;
+            const int groups = 8 * ((nI + 7) / 8) * nJ;
+            hipLaunchKernelGGL(minplus_sparse_kernel<16>, dim3(groups), dim3(kStepWaves * 64), 0,
+                               s, Hp, la.rows, la.lst, la.dense, out, N, nI, nJ);
+        }
+        NSX_HIP(hipGetLastError(), "sparse min-plus launch");
+        return kOk;
+    }
     const int tiles = (N + kTile - 1) / kTile;
     hipLaunchKernelGGL(minplus_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, Hp, out, N,
                        negzero, (const unsigned *)la.dense);
@@ -638,7 +654,17 @@ int negzero_flag(const float *A, int N, unsigned **flag, hipStream_t s) {
 int step_setup(const float *A, int N, hipStream_t s, unsigned **negzero, Lists &la) {
     int rc = negzero_flag(A, N, negzero, s);
     if (rc) return rc;
-    return build_lists(A, N, kWsApspRowsA, kWsApspCandA, true, s, la);
+    rc = build_lists(A, N, kWsApspRowsA, kWsApspCandA, true, s, la);
+    if (rc) return rc;
+    // One readback per call: each of the L steps then launches only the
+    // kernel that does the work (a launch that exits at once still costs
+    // a few microseconds of a step of about ten at N = 1000).
+    unsigned h_dense = 0;
+    NSX_HIP(hipMemcpyAsync(&h_dense, la.dense, sizeof(unsigned), hipMemcpyDeviceToHost, s),
+            "dense flag readback");
+    NSX_HIP(hipStreamSynchronize(s), "dense flag sync");
+    la.host_dense = h_dense != 0;
+    return kOk;
 }
 
 }  // namespace
