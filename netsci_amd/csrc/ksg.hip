@@ -953,12 +953,19 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 ba.groups = (int)std::max<long long>(
                     1, std::min<long long>((n + rows_per_pass - 1) / rows_per_pass,
                                            (8LL * cu_count() + count - 1) / std::max(count, 1)));
+                ba.nodes = count;
+                // Rows of more than 2048 frames are dealt from one range over
+                // all the launch's nodes: 64 workgroups per CU in turn.
+                const long long grid =
+                    width <= 32 ? (long long)count * ba.groups
+                                : std::max<long long>(1, std::min<long long>(
+                                      ((long long)count * n + rows_per_pass - 1) / rows_per_pass,
+                                      64LL * cu_count()));
                 ba.ord = L.ord + (long long)base * slot_entries;
                 ba.rank = L.rank + (long long)base * slot_entries;
                 ba.span = span;
                 ba.abort = abort_flag;
-                hipLaunchKernelGGL(buildk, dim3((unsigned)((long long)count * ba.groups)),
-                                   dim3(ksgw::kWave * block_waves),
+                hipLaunchKernelGGL(buildk, dim3((unsigned)grid), dim3(ksgw::kWave * block_waves),
                                    ksgw::build_lds_bytes(pb.d, n, rs), stream, ba);
                 NSX_HIP(hipGetLastError(), "list build launch");
                 if (b1) (void)hipEventRecord(b1, stream);

@@ -79,7 +79,8 @@ struct BuildArgs {
     int n;              // frames
     long long rs;       // row stride of the tables (entries, a multiple of 8)
     const int *node_of; // table slot -> node
-    int groups;         // workgroups per node
+    int groups;         // workgroups per node (rows of <= 2048 frames)
+    int nodes;          // nodes (table slots) of the launch
     uint16_t *ord;      // [slot][i][rs] frames sorted by (D, j)
     uint16_t *rank;     // [slot][i][rs] upper-bound ranks, indexed by frame
     const float *span;  // per node: max over dimensions of (max - min) of the
@@ -146,8 +147,10 @@ inline int build_width(int n) {
     return n <= 512 ? 8 : n <= 1024 ? 16 : n <= 2048 ? 32 : n <= 4096 ? 64 : n <= 6144 ? 96 : 128;
 }
 inline int build_row_waves(int width) { return width <= 32 ? 1 : width / 32; }
-// Waves per build workgroup: 4 (4, 2 or 1 rows), and 6 for rows of 3 waves.
-constexpr int build_block_waves(int width) { return width == 96 ? 6 : kBuildWaves; }
+// Waves per build workgroup: 4 (4, 2 or 1 rows), and 3 for rows of 3 waves
+// (a workgroup of 6 waves could not share a CU with a second one: its
+// waves fill the SIMDs unevenly, 2-2-1-1, and 3 per SIMD is the limit).
+constexpr int build_block_waves(int width) { return width == 96 ? 3 : kBuildWaves; }
 
 // LDS bytes of the two kernels.
 inline size_t build_lds_bytes(int d, int n, long long rs) {
@@ -377,7 +380,6 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
     const int slot = blockIdx.x / a.groups;
     const int g = blockIdx.x - slot * a.groups;
     const float *x = a.X + (long long)a.node_of[slot] * a.ld;
-    const int lane = threadIdx.x & (kWave - 1);
     const int wib = threadIdx.x / kWave;
     if (a.abort && *a.abort) return;
     const unsigned cbase = build_code_base<21>(a.span[a.node_of[slot]]);
@@ -576,13 +578,13 @@ __device__ __forceinline__ void wide_merges32(unsigned (&v)[32], int lane, int w
 // stages go through LDS. The code t = value >> 13 is monotone in the key
 // (equal codes only within 2^-15 relative, or both more than 16 binades
 // below the node's span) and j < 8192 fits the low bits. Only elements that
-// share t with a neighbour
-// need their exact key bits (ek): odd-even transposition on (ek, j) inside
-// those runs, across the row's waves, restores exact (key, j) order, and
-// the upper-bound ranks come from the ends of runs of equal (t, ek). A
-// workgroup of build_block_waves(32 W) waves holds that many / W rows of one
-// node; coordinates are read from global memory (the node's row stays in
-// cache).
+// share t with a neighbour need their exact key bits (ek): odd-even
+// transposition on (ek, j) inside those runs, across the row's waves,
+// restores exact (key, j) order, and the upper-bound ranks come from the
+// ends of runs of equal (t, ek). A workgroup of build_block_waves(32 W)
+// waves holds that many / W rows at a time, taken in turn from all the
+// launch's rows; coordinates are read from global memory (a node's rows
+// are built by many workgroups at once, so its coordinates stay in L2).
 template <int D, int W>
 __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_build_wide_kernel(
     BuildArgs a) {
@@ -594,18 +596,18 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
     constexpr int kRows = kBW / W;
     constexpr unsigned kPad = 0xffffffffu;  // above every packed finite key
     const int n = a.n;
-    const int slot = blockIdx.x / a.groups;
-    const int g = blockIdx.x - slot * a.groups;
-    const float *x = a.X + (long long)a.node_of[slot] * a.ld;
     const int lane = threadIdx.x & (kWave - 1);
     const int wib = threadIdx.x / kWave;
     const int rg = wib / W, wl = wib - rg * W;  // row in the workgroup, block in the row
     const int ebase = wl * 2048;
     if (a.abort && *a.abort) return;
-    const unsigned cbase = build_code_base<19>(a.span[a.node_of[slot]]);
     unsigned *xr = xch + rg * (W * 2048);
     uint16_t *st = reinterpret_cast<uint16_t *>(xr);  // rank staging after the sort
-    for (int base = g * kRows; base < n; base += a.groups * kRows) {
+    // Rows t = slot * n + i of all the launch's nodes, dealt to the
+    // workgroups in turn, so every workgroup gets the same number of rows
+    // (within one) whatever the node count.
+    const int total = a.nodes * n;  // below 2^31: nodes <= table slots, n <= 8192
+    for (int base = blockIdx.x * kRows; base < total; base += gridDim.x * kRows) {
         // The lane index is re-derived every row from a value the compiler
         // cannot see through, so per-lane addresses and bounds are computed
         // where they are used instead of being hoisted out of the loop and
@@ -613,8 +615,12 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
         int tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const int lane = tid & (kWave - 1);
-        const int i = base + rg;
-        const bool row = i < n;
+        const int tr = base + rg;
+        const bool row = tr < total;
+        const int slot = row ? tr / n : 0;
+        const int i = row ? tr - slot * n : 0;
+        const float *x = a.X + (long long)a.node_of[slot] * a.ld;
+        const unsigned cbase = build_code_base<19>(a.span[a.node_of[slot]]);
         float q[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) q[d] = row ? x[d * a.dstride + i] : 0.0f;
