@@ -950,9 +950,12 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 ba.node_of = L.node_of + first;
                 const int block_waves = ksgw::build_block_waves(width);
                 const int rows_per_pass = block_waves / ksgw::build_row_waves(width);
+                // Rows of up to 2048 frames: groups of rows per node, about 32
+                // workgroups per CU in all (each stages its node's coordinates
+                // in LDS), so the last round of workgroups is a small part.
                 ba.groups = (int)std::max<long long>(
                     1, std::min<long long>((n + rows_per_pass - 1) / rows_per_pass,
-                                           (8LL * cu_count() + count - 1) / std::max(count, 1)));
+                                           (32LL * cu_count() + count - 1) / std::max(count, 1)));
                 ba.nodes = count;
                 // Rows of more than 2048 frames are dealt from one range over
                 // all the launch's nodes: 64 workgroups per CU in turn.
