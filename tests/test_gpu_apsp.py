@@ -118,6 +118,26 @@ def test_paths_unaligned_buffer_bitexact(L):
     np.testing.assert_array_equal(P, Po)
 
 
+@pytest.mark.parametrize("L", [3, 4])
+def test_sparse_steps_long_lists(L):
+    """A sparse graph near the dense threshold (N = 1900, about 115 entries
+    per neighbour list, 16-wide j tiles): lengths (even and odd L, both
+    ping-pong parities) and snapshots must equal the oracle."""
+    N = 1900
+    print(f"apsp sparse steps, long lists N={N} L={L}", flush=True)
+    A = _graph(N, 0.06, seed=19 + L)
+    assert ((A < np.inf).sum(axis=1) + 3) // 4 * 4 @ np.ones(N) / N < 128  # sparse path
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    abi.hedetniemi_lengths(dA, N, L, dH)
+    Ho, Hio, _ = oracle.hedetniemi(A, L, 1e-4, want_hi=True, want_paths=False)
+    np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    abi.hedetniemi(dA, N, L, 1e-4, dH, dHi, None)
+    np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
+    np.testing.assert_array_equal(dHi.numpy().view(np.uint32), Hio.view(np.uint32))
+
+
 def test_lengths_only_matches():
     N, L = 150, 9
     print("apsp lengths", flush=True)
