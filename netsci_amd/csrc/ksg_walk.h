@@ -285,6 +285,49 @@ __device__ __forceinline__ unsigned xor_lane_any(unsigned v) {
     }
 }
 
+// DPP control of the partner lane for lane ^ M (M = 1, 2, 8) or the
+// mirrored lane (M = 3, 7, 15: quad, half-row and row mirrors); 0 where
+// none exists.
+template <int M>
+constexpr int dpp_ctrl() {
+    return M == 1 ? 0xB1 : M == 2 ? 0x4E : M == 3 ? 0x1B : M == 7 ? 0x141 : M == 8 ? 0x128
+         : M == 15 ? 0x140 : 0;
+}
+
+// DPP read of src from the lane CTRL names, with bound_ctrl (lanes whose
+// source is outside the row read 0). Each call is its own instruction, so
+// the compiler folds it into the one v_min_u32 / v_max_u32 that uses it
+// (v_min_u32_dpp): a compare-exchange across lanes is three instructions
+// (min, max, select) instead of four with a separate v_mov_b32_dpp.
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_read(unsigned src) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)src, CTRL, 0xF, 0xF, true);
+}
+
+// min and max of v with DPP reads (CTRL_LO, CTRL_HI) of src, both
+// computed with every lane active: a DPP read of a lane that is switched
+// off returns 0, so the compiler must not move them into the two arms of
+// a branch on `lower` (the empty asm makes both values needed before it).
+template <int CTRL_LO, int CTRL_HI>
+__device__ __forceinline__ unsigned dpp_cmpx(unsigned v, unsigned src, bool lower) {
+    unsigned lo = min(v, dpp_read<CTRL_LO>(src));
+    unsigned hi = max(v, dpp_read<CTRL_HI>(src));
+    asm volatile("" : "+v"(lo), "+v"(hi));
+    return lower ? lo : hi;
+}
+
+// lower ? min(v, p) : max(v, p), p = src of the partner lane of M.
+template <int M>
+__device__ __forceinline__ unsigned lane_cmpx(unsigned v, unsigned src, bool lower) {
+    constexpr int C = dpp_ctrl<M>();
+    if constexpr (C != 0) {
+        return dpp_cmpx<C, C>(v, src, lower);
+    } else {
+        const unsigned o = xor_lane_any<M>(src);
+        return lower ? min(v, o) : max(v, o);
+    }
+}
+
 // Half-cleaner stages STRIDE, STRIDE/2, .., 1 (all ascending). Partner
 // lanes at distance 16 or 32: v_permlane16/32_swap of two registers puts
 // both members of each pair of either register in one lane (the lower
@@ -317,20 +360,16 @@ __device__ __forceinline__ void asc_stages(unsigned (&v)[R], int lane) {
                 }
             }
         } else if constexpr (LX == 4) {
+            // row_shl:4 (lane + 4) for the lower lanes, row_shr:4 (lane - 4)
+            // for the upper ones; the lanes whose read leaves the row take
+            // the other branch.
             const bool lower = (lane & 4) == 0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const unsigned up = (unsigned)__builtin_amdgcn_mov_dpp((int)v[r], 0x104, 0xF, 0xF, false);
-                const unsigned dn = (unsigned)__builtin_amdgcn_mov_dpp((int)v[r], 0x114, 0xF, 0xF, false);
-                v[r] = lower ? min(v[r], up) : max(v[r], dn);
-            }
+            for (int r = 0; r < R; ++r) v[r] = dpp_cmpx<0x104, 0x114>(v[r], v[r], lower);
         } else {
             const bool lower = (lane & LX) == 0;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const unsigned o = xor_lane<LX>(v[r]);
-                v[r] = lower ? min(v[r], o) : max(v[r], o);
-            }
+            for (int r = 0; r < R; ++r) v[r] = lane_cmpx<LX>(v[r], v[r], lower);
         }
     } else {
 #pragma unroll
@@ -357,14 +396,16 @@ __device__ __forceinline__ void mirror_merges(unsigned (&v)[R], int lane) {
             v[r2] = max(x, y);
         }
     } else {
-        // Element (lane, r) pairs with (lane ^ M, R - 1 - r).
+        // Element (lane, r) pairs with (lane ^ M, R - 1 - r); registers r
+        // and R - 1 - r are updated together from their old values.
         constexpr int M = SIZE / R - 1;
         const bool lower = (lane & ((M + 1) >> 1)) == 0;
-        unsigned o[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) o[r] = xor_lane_any<M>(v[R - 1 - r]);
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = lower ? min(v[r], o[r]) : max(v[r], o[r]);
+        for (int r = 0; r < R / 2; ++r) {
+            const unsigned x = v[r], y = v[R - 1 - r];
+            v[r] = lane_cmpx<M>(x, y, lower);
+            v[R - 1 - r] = lane_cmpx<M>(y, x, lower);
+        }
     }
     if constexpr (SIZE >= 4) asc_stages<R, SIZE / 4>(v, lane);
     if constexpr (SIZE < kWave * R) mirror_merges<R, SIZE * 2>(v, lane);
@@ -673,37 +714,39 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
             if ((r & 7) == 7) asm volatile("" ::: "memory");
         }
         // Odd-even transposition within runs, until no pair of the
-        // workgroup's rows swaps. A pair with different codes is in
-        // order already.
-        auto cx = [](unsigned &ux, unsigned &kx, unsigned &uy, unsigned &ky) {
-            const bool sw = (ux >> 13) == (uy >> 13) &&
-                            (kx > ky || (kx == ky && (ux & 0x1fffu) > (uy & 0x1fffu)));
-            const unsigned tu = ux, tk = kx;
-            ux = sw ? uy : ux;
-            kx = sw ? ky : kx;
-            uy = sw ? tu : uy;
-            ky = sw ? tk : ky;
+        // workgroup's rows swaps. An element is (ek << 32) | u as one 64-bit
+        // value: inside a run (equal codes) u orders like j, so (ek, u)
+        // orders like the exact (key, j), and one 64-bit compare decides a
+        // swap; a pair with different codes is in order already.
+        unsigned long long K[32];
+#pragma unroll
+        for (int r = 0; r < 32; ++r) K[r] = ((unsigned long long)ek[r] << 32) | u[r];
+        auto cx = [](unsigned long long &x, unsigned long long &y) {
+            const bool sw = ((unsigned)x ^ (unsigned)y) < 0x2000u && x > y;
+            const unsigned long long t = x;
+            x = sw ? y : x;
+            y = sw ? t : y;
             return sw;
         };
         for (;;) {
             bool swapped = false;
 #pragma unroll
-            for (int r = 0; r + 1 < 32; r += 2) swapped |= cx(u[r], ek[r], u[r + 1], ek[r + 1]);
+            for (int r = 0; r + 1 < 32; r += 2) swapped |= cx(K[r], K[r + 1]);
 #pragma unroll
-            for (int r = 1; r + 1 < 32; r += 2) swapped |= cx(u[r], ek[r], u[r + 1], ek[r + 1]);
+            for (int r = 1; r + 1 < 32; r += 2) swapped |= cx(K[r], K[r + 1]);
             if (threadIdx.x == 0) any_swap = 0;
             if (lane == 0) {
-                edge_u[0][wib] = u[0];
-                edge_k[0][wib] = ek[0];
+                edge_u[0][wib] = (unsigned)K[0];
+                edge_k[0][wib] = (unsigned)(K[0] >> 32);
             }
             if (lane == kWave - 1) {
-                edge_u[1][wib] = u[31];
-                edge_k[1][wib] = ek[31];
+                edge_u[1][wib] = (unsigned)K[31];
+                edge_k[1][wib] = (unsigned)(K[31] >> 32);
             }
-            unsigned nu = (unsigned)__shfl_down((int)u[0], 1);
-            unsigned nk = (unsigned)__shfl_down((int)ek[0], 1);
-            unsigned pu = (unsigned)__shfl_up((int)u[31], 1);
-            unsigned pk = (unsigned)__shfl_up((int)ek[31], 1);
+            unsigned nu = (unsigned)__shfl_down((int)(unsigned)K[0], 1);
+            unsigned nk = (unsigned)__shfl_down((int)(unsigned)(K[0] >> 32), 1);
+            unsigned pu = (unsigned)__shfl_up((int)(unsigned)K[31], 1);
+            unsigned pk = (unsigned)__shfl_up((int)(unsigned)(K[31] >> 32), 1);
             __syncthreads();
             if (lane == kWave - 1) {
                 nu = wl + 1 < W ? edge_u[0][wib + 1] : kPad;
@@ -716,22 +759,23 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
             // (lane, 31) keeps the smaller of itself and its successor,
             // (lane, 0) the larger of its predecessor and itself.
             {
-                unsigned au = u[31], ak = ek[31], bu = nu, bk = nk;
-                swapped |= cx(au, ak, bu, bk);
-                u[31] = au;
-                ek[31] = ak;
+                unsigned long long nx = ((unsigned long long)nk << 32) | nu;
+                swapped |= cx(K[31], nx);
             }
             {
-                unsigned au = pu, ak = pk, bu = u[0], bk = ek[0];
-                cx(au, ak, bu, bk);
-                u[0] = bu;
-                ek[0] = bk;
+                unsigned long long px = ((unsigned long long)pk << 32) | pu;
+                cx(px, K[0]);
             }
             if (__ballot(swapped) && lane == 0) atomicOr(&any_swap, 1);
             __syncthreads();
             const int again = any_swap;
             __syncthreads();
             if (!again) break;
+        }
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            u[r] = (unsigned)K[r];
+            ek[r] = (unsigned)(K[r] >> 32);
         }
 
         // ord row.
