@@ -278,6 +278,14 @@ def set_table_budget(nbytes=0):
     check(lib().nsx_ksg_set_table_budget(int(nbytes)), "set table budget")
 
 
+def table_slot_bytes(n):
+    """Device bytes of one node's list-path tables for windows of n frames:
+    the ord and rank tables (rows of n rounded up to 8, 2 bytes per entry)
+    and one float prior per row (ksg.hip's slot_bytes_for)."""
+    rows = (n + 7) // 8 * 8
+    return 2 * rows * rows * 2 + rows * 4
+
+
 def release_workspace():
     """Free the library's device workspaces and tables."""
     check(lib().nsx_release_workspace(), "release workspace")

@@ -555,6 +555,7 @@ struct ListLayout {
     const int *ab = nullptr;    // pair list the walks index (pb.ab or ab_perm)
     const int *perm = nullptr;  // nullable: output index of each listed pair
     uint16_t *ord = nullptr, *rank = nullptr;
+    float *prior = nullptr;  // [slot][table_rows(n)] list-choice priors
     unsigned *cnt = nullptr, *tie_list = nullptr;
     long long chunk = 0;   // pairs per walk launch (packed counts below 4 GiB)
     unsigned tie_cap = 0;
@@ -594,7 +595,7 @@ int walk_shape(const Problem &pb, WalkShape &ws) {
     ws.tieo = tie_for(pb.d, pb.k, ksgw::kTieOverflow);
     NSX_REQUIRE(ws.walk && ws.ties && ws.tiek && ws.tieo, "ksg: no list-path kernel for (d, k)");
     const size_t lds_cap = 160 * 1024;
-    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    const size_t rec_bytes = ksgw::walk_buffer_bytes(pb.d, pb.nmax);
     ws.wpb = std::min(g_walk_wpb, ksgw::walk_max_waves(pb.k));
     int per_cu = 0;
     const int nbuf_max = (int)std::min<size_t>(3, (lds_cap - 256) / rec_bytes);
@@ -606,11 +607,14 @@ int walk_shape(const Problem &pb, WalkShape &ws) {
     return kOk;
 }
 
-// Bytes of one node slot of the tables (ord and rank) for windows of up to
-// nmax frames.
-size_t slot_bytes_for(int nmax) {
+// Bytes of one node slot's ord (or rank) table, and of the whole slot (ord,
+// rank and the per-frame priors), for windows of up to nmax frames.
+size_t table_bytes_for(int nmax) {
     const long long rs = ((long long)nmax + 7) & ~7LL;
-    return (size_t)2 * (size_t)ksgw::table_rows(nmax) * (size_t)rs * sizeof(uint16_t);
+    return (size_t)ksgw::table_rows(nmax) * (size_t)rs * sizeof(uint16_t);
+}
+size_t slot_bytes_for(int nmax) {
+    return 2 * table_bytes_for(nmax) + (size_t)ksgw::table_rows(nmax) * sizeof(float);
 }
 
 // The schedule for U used nodes in S slots (see above). sched_of[x * nb + y]
@@ -782,8 +786,10 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok,
                         "ksg: no device memory for two nodes' list-path tables");
             return kOk;
         }
+        const long long tab_entries = (long long)(table_bytes_for(pb.nmax) / sizeof(uint16_t));
         L.ord = (uint16_t *)tab;
-        L.rank = L.ord + L.S * (long long)(sb / 2 / sizeof(uint16_t));
+        L.rank = L.ord + L.S * tab_entries;
+        L.prior = reinterpret_cast<float *>(L.rank + L.S * tab_entries);
         make_schedule(L, sched_of);
     }
 
@@ -915,7 +921,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
         const int width = ksgw::build_width(n);
         const ksgw::BuildFn buildk = build_for(pb.d, width);
         NSX_REQUIRE(buildk, "ksg: no list-path table kernel for this frame count");
-        const size_t wrec = (size_t)2 * pb.d * n * sizeof(float);
+        const size_t wrec = ksgw::walk_buffer_bytes(pb.d, n);
         const int nbuf = (int)std::min<size_t>(3, (lds_cap - 256) / wrec);
         const size_t wlds = ksgw::walk_lds_bytes(pb.d, n, nbuf);
         const long long wchunk = std::max<long long>(
@@ -966,6 +972,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                                       64LL * cu_count()));
                 ba.ord = L.ord + (long long)base * slot_entries;
                 ba.rank = L.rank + (long long)base * slot_entries;
+                ba.prior = L.prior + (long long)base * ksgw::table_rows(n);
                 ba.span = span;
                 ba.abort = abort_flag;
                 hipLaunchKernelGGL(buildk, dim3((unsigned)grid), dim3(ksgw::kWave * block_waves),
@@ -991,6 +998,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 wa.k = pb.k;
                 wa.ord = L.ord;
                 wa.rank = L.rank;
+                wa.prior = L.prior;
                 wa.rs = rs;
                 wa.slot_of = L.nb > 1 ? L.rslot_of : L.slot_of;
                 wa.cnt = L.cnt;
@@ -1087,7 +1095,7 @@ int run_lists(const Problem &pb, const float *d_psi, float *mi_out, int *counts_
     *ran = false;
     if (g_method == 1 && pb.k <= ksg::kMaxK) return kOk;
     const size_t lds_cap = 160 * 1024;
-    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    const size_t rec_bytes = ksgw::walk_buffer_bytes(pb.d, pb.nmax);
     const bool fits = pb.k > ksg::kMaxK ? ksgw::bigk_lds_bytes(pb.nmax) <= lds_cap
                                         : rec_bytes + 256 <= lds_cap;
     const bool eligible = pb.nmax <= ksgw::kMaxFrames && fits &&
@@ -1161,7 +1169,7 @@ int run_chunked(const Problem &pb, float *mi_out, float *h_mi_out, const long lo
     bool lists = false;
     ListLayout L;
     WalkShape ws;
-    const size_t rec_bytes = (size_t)2 * pb.d * pb.nmax * sizeof(float);
+    const size_t rec_bytes = ksgw::walk_buffer_bytes(pb.d, pb.nmax);
     const bool fits = pb.k > ksg::kMaxK ? ksgw::bigk_lds_bytes(pb.nmax) <= 160 * 1024
                                         : rec_bytes + 256 <= 160 * 1024;
     if ((g_method != 1 || pb.k > ksg::kMaxK) && pb.nmax <= ksgw::kMaxFrames && fits) {

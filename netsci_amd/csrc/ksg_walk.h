@@ -9,39 +9,41 @@
 //     be sorted once per call, per query frame i, by (D_m(i, j), j) — the
 //     sorted frame list ord_m[i][.] — together with the upper-bound rank
 //     rank_m[i][j] = #{j' : D_m(i, j') <= D_m(i, j)};
-//   * the joint distance is z(i, j) = max(D_a(i, j), D_b(i, j)), so walking
-//     ord_a[i] and ord_b[i] (in any interleaving) and keeping the k+1
-//     smallest z of the distinct frames seen so far, every frame not yet seen
-//     in either list has z >= max(last D_a walked, last D_b walked). Once that
-//     bound exceeds the current (k+1)-th smallest z, the k+1 smallest are
-//     final (the threshold-algorithm stopping rule). The k+1 nearest lie
-//     within both lists' walked prefixes, so walking mostly the list whose
-//     keys grow fastest is enough.
+//   * the joint distance is z(i, j) = max(D_a(i, j), D_b(i, j)) >= D_a(i, j),
+//     so walking ord_a[i] alone and keeping the k+1 smallest z of the frames
+//     walked, every frame not walked yet has z >= the last D_a walked. Once
+//     that bound exceeds the current (k+1)-th smallest z, the k+1 smallest
+//     are final (the threshold-algorithm stopping rule, with one list). The
+//     k+1 nearest lie within either list's walked prefix, so either list
+//     gives the same result; the walk takes the one whose keys grow faster
+//     around the query (the larger "prior": the marginal distance at list
+//     position 64, which the table build records per node and frame), which
+//     is the shorter walk for about 97% of the queries of a C-alpha
+//     trajectory.
 //
 // The marginal counts then need no second pass: with eps_x = D_a(i, j*) for
 // the selected frame j* of largest D_a, n_x + 1 = #{j : D_a(i, j) <= eps_x}
 // = rank_a[i][j*] (self included, as the reference counts it before
 // subtracting one), and likewise n_y.
 //
-// A frame seen in one list is recognised in the other without memory: ord_m
-// is sorted by the pair (D_m, j), so frame j has already been walked in list
-// m exactly when (D_m(i, j), j) <= the last walked (D_m, j) in that order.
-//
 // Exact ties at the k-th / (k+1)-th joint distance take the libstdc++
 // std::sort-order selection of sort_select.h, as in ksg_kernel.h.
 //
 // Mapping:
-//   * list build: one wavefront per (node, query frame) row: the n keys are
-//     packed as (float bits << 32 | j) and bitonic-sorted in registers
-//     (32 per lane, cross-lane stages by ds_bpermute); ranks come from a
-//     suffix minimum over the ends of equal-key runs;
-//   * walk: a workgroup owns one node pair at a time with both nodes'
+//   * list build: one wavefront per (node, query frame) row (up to four
+//     for rows of more than 2048 frames): the n keys are packed as 32-bit
+//     (distance code << id bits | j) and bitonic-sorted in registers, an
+//     odd-even pass on exact keys restores (D, j) order inside runs of
+//     equal codes; ranks come from a suffix minimum over the ends of
+//     equal-key runs, and the key at position 64 is the frame's prior;
+//   * walk: a workgroup owns a few node pairs at a time with both nodes'
 //     coordinates in LDS, one 2d-float record per frame (gathered at random
-//     frames with 8-byte loads); its lanes take query
-//     frames from an LDS counter, each lane walking its own query in bodies
-//     of 8 steps (queries start at a body boundary, so every body reads one
-//     aligned 8-entry block per list at fixed register positions, and the
-//     block two ahead is requested one body before it is needed). Lanes
+//     frames with 8-byte loads) and one list-choice bit per frame; its lanes
+//     take query frames from an LDS counter, each lane walking its own query
+//     in bodies of 8 steps of its one list (queries start at a body
+//     boundary, so every body reads one aligned 8-entry block at fixed
+//     register positions, and the block two ahead is requested one body
+//     before it is needed). Lanes
 //     whose walk ended wait until `batch` of them have (or no lane walks),
 //     then the wave processes them together (eps, ties, rank lookups, whose
 //     counts are stored one round later) and hands out new queries, so that
@@ -87,6 +89,8 @@ struct BuildArgs {
                         // coordinate, a bound of every marginal distance
     const unsigned *abort;  // nullable: return at once when *abort != 0 (a
                             // non-finite coordinate was found beforehand)
+    float *prior;       // [slot][table_rows(n)]: the marginal distance at list
+                        // position prior_pos(n), the walk's choice of list
 };
 
 // Packed counts of a tied query the tie list had no room for: the counts
@@ -119,6 +123,7 @@ struct WalkArgs {
     int code_shift;             // float-bit units per distance code step: 2^code_shift
     const unsigned *abort;      // nullable: the kernels return at once when *abort != 0
                                 // (a non-finite coordinate; the all-frames kernel runs)
+    const float *prior;         // [slot][table_rows(n)] (BuildArgs::prior)
 };
 
 // Frame-order psi sums of the packed counts (one lane per pair). Local pair
@@ -157,8 +162,13 @@ inline size_t build_lds_bytes(int d, int n, long long rs) {
     if (n > 2048) return (size_t)build_block_waves(build_width(n)) * 2048 * 4;  // exchange rows
     return (((size_t)d * n * 4 + 15) & ~(size_t)15) + (size_t)kBuildWaves * rs * 2;
 }
+// LDS bytes of one staged pair of the walk: both nodes' coordinates per
+// frame and one list-choice bit per frame.
+inline size_t walk_buffer_bytes(int d, int n) {
+    return (size_t)2 * d * n * 4 + (size_t)4 * ((n + 31) / 32);
+}
 inline size_t walk_lds_bytes(int d, int n, int nbuf) {
-    return (size_t)nbuf * 2 * d * n * 4 + (size_t)(8 + 5 * nbuf) * sizeof(int);
+    return (size_t)nbuf * walk_buffer_bytes(d, n) + (size_t)(8 + 5 * nbuf) * sizeof(int);
 }
 
 // Table layout: the rows of 8 consecutive frames of a node are stored
@@ -177,6 +187,17 @@ NSX_HD inline long long table_row(int slot, int i, int n, long long rs) {
 }
 NSX_HD inline long long table_at(long long row, long long e) {
     return row + (e >> 3) * (8 * kRowGroup) + (e & 7);
+}
+
+// The list position whose marginal distance (the "prior" of a node's frame)
+// tells how fast the frame's list grows: the walk takes, per query, the list
+// whose prior is larger, i.e. whose prefix below the query's k+1 nearest is
+// expected to be shorter. Position 64, or for rows of at most 64 frames the
+// last multiple of the build's per-lane width R below n, so that it is
+// register 0 of lane prior_pos / R in the builds.
+constexpr int kPriorPos = 64;
+NSX_HD inline int prior_pos(int n, int R) {
+    return n > kPriorPos ? kPriorPos : ((n - 1) / R) * R;
 }
 
 // LDS bytes per wave of the tie kernels: the row (n records), the two
@@ -514,6 +535,7 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
             v[r] = (unsigned)ex[r] & 0x7ffu;
             kx[r] = ex[r] == ~0ull ? __builtin_inff() : __uint_as_float((unsigned)(ex[r] >> 32));
         }
+        if (lane == prior_pos(n, R) / R) a.prior[(long long)slot * table_rows(n) + i] = kx[0];
 
         // ord row: this lane's R consecutive positions, 16 bytes at a time.
         const long long orow = table_row(slot, i, n, a.rs);
@@ -777,6 +799,10 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
             u[r] = (unsigned)K[r];
             ek[r] = (unsigned)(K[r] >> 32);
         }
+        // The prior (position 64 = register 0 of lane 2 of the row's first
+        // wave), from its code: within 2^-15 relative of the key.
+        if (row && wl == 0 && lane == kPriorPos / 32)
+            a.prior[(long long)slot * table_rows(n) + i] = __uint_as_float(((u[0] >> 13) << 8) + cbase);
 
         // ord row.
         const long long orow = table_row(slot, row ? i : 0, n, a.rs);
@@ -883,16 +909,6 @@ __device__ __forceinline__ unsigned key_code(float key, unsigned base, int shift
     return __builtin_elementwise_sub_sat(__float_as_uint(key), base) >> shift;
 }
 
-// (D, j) as one signed 64-bit value ordered like the list's (D, j) order:
-// D >= 0 is finite, so its bits compare like the float; -1 sorts first.
-__device__ __forceinline__ long long lex(float key, int j) {
-    return (long long)(((unsigned long long)__float_as_uint(key) << 32) | (unsigned)j);
-}
-
-__device__ __forceinline__ float lex_key(long long v) {
-    return __uint_as_float((unsigned)((unsigned long long)v >> 32));
-}
-
 // Frame j of the pair's LDS record array: 2D floats per frame (node a's D
 // coordinates, then node b's), read as D 8-byte LDS loads.
 template <int D>
@@ -912,11 +928,6 @@ __device__ __forceinline__ void fetch(const float *rec, int j, float (&va)[D], f
     }
 }
 
-// Component-wise select (a select of whole vectors can go through memory).
-__device__ __forceinline__ uint4 sel4(bool c, uint4 x, uint4 y) {
-    return make_uint4(c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z, c ? x.w : y.w);
-}
-
 __device__ __forceinline__ uint4 load_block(const uint16_t *p) {
     return *reinterpret_cast<const uint4 *>(p);
 }
@@ -934,13 +945,14 @@ __device__ __forceinline__ int ring_slot(int s, int nbuf) {
 }
 
 // Stage local pair s of this workgroup into ring buffer s % nbuf: both rows'
-// coordinates as per-frame records, the two nodes' table slots and the
-// pair's code base: the larger span's float bits minus the code window's
+// coordinates as per-frame records, one list-choice bit per frame (1: node
+// a's list, whose prior is at least node b's), the two nodes' table slots and
+// the pair's code base: the larger span's float bits minus the code window's
 // width less one (at least 0), so that every key's code fits 32 - id_bits
 // bits.
 template <int D>
-__device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *meta, long long s,
-                                           int n) {
+__device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, unsigned *choice0,
+                                           int *meta, long long s, int n) {
     const long long p = a.p0 + (long long)blockIdx.x + s * (long long)gridDim.x;
     const int b = ring_slot((int)s, a.nbuf);
     const int na = a.ab[2 * p], nb = a.ab[2 * p + 1];
@@ -951,6 +963,22 @@ __device__ __forceinline__ void stage_pair(const WalkArgs &a, float *rec0, int *
         const int d = t / n, f = t - d * n;
         rec[2 * D * f + d] = xa[d * a.dstride + f];
         rec[2 * D * f + D + d] = xb[d * a.dstride + f];
+    }
+    const long long rows = table_rows(n);
+    const float *pa = a.prior + (long long)a.slot_of[na] * rows;
+    const float *pb = a.prior + (long long)a.slot_of[nb] * rows;
+    const int cwords = (n + 31) / 32;
+    unsigned *choice = choice0 + (long long)b * cwords;
+    // blockDim.x is a multiple of 64: each wave's ballot covers 64 frames
+    // from a multiple of 64, i.e. two choice words.
+    for (int f0 = 0; f0 < n; f0 += blockDim.x) {
+        const int f = f0 + (int)threadIdx.x;
+        const unsigned long long m = __ballot(f < n && pa[f] >= pb[f]);
+        const int w = (f0 + ((int)threadIdx.x & ~(kWave - 1))) / 32;
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            if (w < cwords) choice[w] = (unsigned)m;
+            if (w + 1 < cwords) choice[w + 1] = (unsigned)(m >> 32);
+        }
     }
     if (threadIdx.x == 0) {
         meta[3 * b] = a.slot_of[na];
@@ -979,10 +1007,13 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     int *ctrl = reinterpret_cast<int *>(lds + nbuf * rec_floats);
     // ctrl[0] next query, [1] staged pairs, [2] retired pairs, [3] pairs to
     // stage now, [8..8+nbuf) done queries per buffer, then per buffer the 2
-    // table slots and the code base, then the pairs to stage.
+    // table slots and the code base, then the pairs to stage, then per
+    // buffer the list-choice bits.
     int *done = ctrl + 8;
     int *meta = done + nbuf;
     int *to_stage = meta + 3 * nbuf;
+    unsigned *choice = reinterpret_cast<unsigned *>(to_stage + nbuf);
+    const int cwords = (n + 31) / 32;
     const int idb = a.id_bits, shift = a.code_shift;
     const unsigned id_mask = (1u << idb) - 1u;
     const int lane = threadIdx.x & (kWave - 1);
@@ -999,7 +1030,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         for (int b = 0; b < nbuf; ++b) done[b] = 0;
     }
     const int first = (int)(npairs < nbuf ? npairs : nbuf);
-    for (int s = 0; s < first; ++s) stage_pair<D>(a, lds, meta, s, n);
+    for (int s = 0; s < first; ++s) stage_pair<D>(a, lds, choice, meta, s, n);
     __syncthreads();
     if (threadIdx.x == 0) ctrl[1] = first;
 
@@ -1009,14 +1040,14 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     bool res = false;  // a finished walk waits for its counts
     int q = 0;   // the lane's query number: frame i = q - qs * n of local pair qs
     int qs = 0;
-    int i = 0, posA = 0, posB = 0;  // entries walked in each list
+    int i = 0, pos = 0;  // entries walked in the lane's list
     int rec = 0;  // LDS float offset of the lane's pair records (an offset, not a
                   // pointer, so every access stays an LDS instruction)
-    long long offA = 0, offB = 0;  // table row offsets of (node, frame i)
-    unsigned base = 0;             // the pair's code base
-    // Last walked entry of each list as lex(D, j); -1: none yet (its key
-    // reads as NaN, so the stop test below is false).
-    long long lastA = -1, lastB = -1;
+    // Table row offsets of (node, frame i): the walked list's and the other
+    // node's (its ranks are looked up at the end).
+    long long off = 0, offO = 0;
+    bool useA = true;  // the walked list is node a's
+    unsigned base = 0; // the pair's code base
     float qa[D], qb[D];
     // The k+1 smallest (code(z) << idb | j) of the frames evaluated, in
     // ascending order. Codes are monotone in z, so when the k-th and
@@ -1024,10 +1055,9 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     // exactly; equal codes (a tie of z, or two z too close for the code)
     // go to the tie list.
     unsigned h[K + 1];
-    // List blocks: cur holds entries [pos, pos + 8) of each list, nxt the 8
-    // after them (each list's pos is a multiple of 8 at every body start,
-    // until the list is exhausted).
-    uint4 curA = {0, 0, 0, 0}, curB = {0, 0, 0, 0}, nxtA = {0, 0, 0, 0}, nxtB = {0, 0, 0, 0};
+    // List blocks: cur holds entries [pos, pos + 8), nxt the 8 after them
+    // (pos is a multiple of 8 at every body start until the list ends).
+    uint4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
     // Counts of the previous refill round, stored one round later so that
     // the rank lookups' latency is not waited for.
     bool pend = false;
@@ -1057,7 +1087,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             __syncthreads();
             const int ns = ctrl[3];
             if (ns > 0) {
-                for (int x = 0; x < ns; ++x) stage_pair<D>(a, lds, meta, to_stage[x], n);
+                for (int x = 0; x < ns; ++x) stage_pair<D>(a, lds, choice, meta, to_stage[x], n);
                 __syncthreads();
                 // Published without a barrier: a wave that reads the old
                 // count starts the new pair's queries one pass later, and
@@ -1120,12 +1150,13 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                     // (0 - 1) | (0 - 1) << 16 is kTieMark.
                     // Non-temporal: each rank line serves one lookup, and
                     // keeping it out of L2 leaves room for the list rows.
+                    const long long offA = useA ? off : offO, offB = useA ? offO : off;
                     pend_x = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(offA, jx));
                     pend_y = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(offB, jy));
                     pend_at = ((long long)blockIdx.x + qs * G) * n + i;
                     pend = true;
                 }
-                steps += (unsigned)(posA + posB);
+                steps += (unsigned)pos;
                 atomicAdd(done + ring_slot(qs, nbuf), 1);  // its LDS records are no longer read
             }
             res = false;
@@ -1147,7 +1178,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                     qs = d;
                 }
             }
-            // (3) start the reserved queries whose pairs are staged.
+            // (3) start the reserved queries whose pairs are staged: the
+            // walk takes the list the staged choice bit names.
             if (state == 3 && q < avail) {
                 const int b = ring_slot(qs, nbuf);
                 i = q - qs * n;
@@ -1155,17 +1187,15 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 fetch<D>(lds + rec, i, qa, qb);
 #pragma unroll
                 for (int m = 0; m <= K; ++m) h[m] = 0xffffffffu;
-                posA = posB = 0;
-                lastA = lastB = -1;
-                offA = table_row(meta[3 * b], i, n, rs);
-                offB = table_row(meta[3 * b + 1], i, n, rs);
+                pos = 0;
+                useA = ((choice[b * cwords + (i >> 5)] >> (i & 31)) & 1u) != 0u;
+                const long long offA = table_row(meta[3 * b], i, n, rs);
+                const long long offB = table_row(meta[3 * b + 1], i, n, rs);
+                off = useA ? offA : offB;
+                offO = useA ? offB : offA;
                 base = (unsigned)meta[3 * b + 2];
-                curA = load_block(a.ord + offA);
-                curB = load_block(a.ord + offB);
-                if (rs > 8) {
-                    nxtA = load_block(a.ord + table_at(offA, 8));
-                    nxtB = load_block(a.ord + table_at(offB, 8));
-                }
+                cur = load_block(a.ord + off);
+                if (rs > 8) nxt = load_block(a.ord + table_at(off, 8));
                 state = 1;
             }
         }
@@ -1173,93 +1203,44 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         // next pass keeps the loop's registers in place), and the next
         // pass is a checkpoint.
         if (__ballot(state == 1) == 0) pass = (pass / a.checkpoint + 1) * a.checkpoint - 1;
-        // A body: the next 8 entries of one list. The lane takes the list
-        // whose walk is expected to end sooner: after the first block of each,
-        // a list's remaining entries below h[k] are estimated as
-        // pos * ((h[k] / last key)^d - 1) (a d-dimensional ball holds about
-        // r^d of the frames). Any choice gives the same result; this one
-        // walks about 40% fewer entries than taking one of each per step.
-        bool useA;
-        {
-            // h[K]'s distance, to within its code step.
-            const float hk = __uint_as_float(((h[K] >> idb) << shift) + base);
-            if (posA >= n) {
-                useA = false;
-            } else if (posB >= n) {
-                useA = true;
-            } else if (posA < 8 || posB < 8 || h[K] == 0xffffffffu) {
-                useA = posA <= posB;
-            } else {
-                // An estimate only: the hardware reciprocal is precise enough.
-                const float ra = hk * __builtin_amdgcn_rcpf(fmaxf(lex_key(lastA), 1e-30f));
-                const float rb = hk * __builtin_amdgcn_rcpf(fmaxf(lex_key(lastB), 1e-30f));
-                float pa = ra, pb = rb;
-#pragma unroll
-                for (int d = 1; d < D; ++d) {
-                    pa *= ra;
-                    pb *= rb;
-                }
-                useA = (float)posA * (pa - 1.0f) <= (float)posB * (pb - 1.0f);
-            }
-        }
-        const uint4 blk = sel4(useA, curA, curB);
-        const long long off = useA ? offA : offB;
-        const long long lastO = useA ? lastB : lastA;  // fixed during the body
-        long long lastL = useA ? lastA : lastB;
-        int posL = useA ? posA : posB;
-        // The chosen list's block two ahead is requested now and used one
-        // body later.
+        // A body: the next 8 entries of the lane's list. The block two ahead
+        // is requested now and used one body later.
         uint4 tmp = {0, 0, 0, 0};  // unused when the list ends before that block
-        if (state == 1 && posL + 16 < rs) tmp = load_block(a.ord + table_at(off, posL + 16));
+        if (state == 1 && pos + 16 < rs) tmp = load_block(a.ord + table_at(off, pos + 16));
         const bool walks = state == 1;
-        const int rem = n - posL;  // entries left in the chosen list
-        float key7 = 0.0f;         // the chosen list's key of the body's last entry
-        int j7 = 0;
+        const int rem = n - pos;  // entries left in the list
+        float key7 = 0.0f;        // the list's key of the body's last entry
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
             // Every lane runs every step (no branches: the loop's registers
             // stay in place); a lane that is not walking inserts nothing.
             // Block entries are frames (the tables' padding past a list's
             // end holds the query frame i, which is never inserted; blocks
-            // of no walk are 0), so every read stays within the lane's
-            // records.
-            const int j = block_entry(blk, st);
+            // of no walk are 0 or a previous walk's), so every read stays
+            // within the lane's records.
+            const int j = block_entry(cur, st);
             float u[D], v[D];
             fetch<D>(lds + rec, j, u, v);
             const float da = marg<D>(qa, u), db = marg<D>(qb, v);
-            // Already evaluated if the other list has walked it.
-            const bool seen = lex(useA ? db : da, j) <= lastO;
             const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
-            const bool take = walks & !seen & (j != i);
+            const bool take = walks & (j != i);
             insert_packed<K>(h, take ? c : 0xffffffffu);
-            if (st == 7) {
-                key7 = useA ? da : db;
-                j7 = j;
-            }
+            if (st == 7) key7 = useA ? da : db;
         }
-        // The stopping rule, once per body: it is monotone (last keys only
-        // grow, h[K] only falls), so a walk that meets it inside the body
-        // meets it here too, and the entries after that point change
-        // nothing (their codes exceed h[K]'s, or they were seen). A list
-        // walked to its end has evaluated every frame. In codes: a larger
-        // code means a larger distance; a missing last key reads as NaN,
-        // which fmaxf ignores.
-        posL += walks ? min(rem, 8) : 0;
-        lastL = walks ? lex(key7, j7) : lastL;
-        const unsigned stop = key_code(fmaxf(key7, lex_key(lastO)), base, shift) << idb;
+        // The stopping rule, once per body: every frame not walked yet has a
+        // key of at least key7 in this list, so a joint distance of at least
+        // key7; once code(key7) exceeds h[K]'s code none of them can enter.
+        // The rule is monotone (keys only grow, h[K] only falls), so a walk
+        // that meets it inside the body meets it here too, and the entries
+        // after that point change nothing. A list walked to its end has
+        // evaluated every frame.
+        pos += walks ? min(rem, 8) : 0;
+        const unsigned stop = key_code(key7, base, shift) << idb;
         const bool fin = walks & ((rem <= 8) | (stop > h[K]));
         res = res | fin;
         state = fin ? 0 : state;
-        lastA = useA ? lastL : lastA;
-        posA = useA ? posL : posA;
-        lastB = useA ? lastB : lastL;
-        posB = useA ? posB : posL;
-        // Lanes that are not walking rotate too: a walk's first blocks are
-        // loaded when it starts.
-        curA = sel4(useA, nxtA, curA);
-        nxtA = sel4(useA, tmp, nxtA);
-        curB = sel4(useA, curB, nxtB);
-        nxtB = sel4(useA, nxtB, tmp);
+        cur = nxt;
+        nxt = tmp;
     }
     if (pend) __builtin_nontemporal_store((pend_x - 1u) | ((pend_y - 1u) << 16), a.cnt + pend_at);
     // Per-wave totals.

@@ -126,8 +126,7 @@ def test_blocked_list_plan():
     dab = abi.DeviceArray.from_numpy(ab)
     want = _windows_eager(dX, X.shape[1], nT, d, win, dab, P, k, abi.METHOD_ALL_FRAMES)
     out = abi.DeviceArray((R, P), np.float32)
-    rows = (nT + 7) // 8 * 8
-    abi.set_table_budget(3 * 2 * rows * rows * 2)
+    abi.set_table_budget(3 * abi.table_slot_bytes(nT))
     try:
         plan = abi.KsgPlan(dX, X.shape[1], nT, d, win, dab, P, k, out, autotune=False)
     finally:

@@ -90,9 +90,8 @@ def test_chunked_pair_loop_values_at_every_chunk(method, budget_slots):
     ab = synth.all_pairs(N)  # 66 pairs
     ends = [1, 11, 21, 31, 41, 51, 61, 66]
     want = oracle.pairs(X, ab, k, n, d, gc=False)
-    rows = (n + 7) // 8 * 8
     abi.set_method(method)
-    abi.set_table_budget(budget_slots * 2 * rows * rows * 2)
+    abi.set_table_budget(budget_slots * abi.table_slot_bytes(n))
     try:
         rc, host, seen = _chunked(X, ab, k, n, d, ends)
     finally:

@@ -29,9 +29,7 @@ def _reset():
 
 
 def _slot_bytes(n):
-    rows = (n + 7) // 8 * 8
-    rs = (n + 7) // 8 * 8
-    return 2 * rows * rs * 2
+    return abi.table_slot_bytes(n)
 
 
 def _mi(X, ab, k, n, d, method, counts=False, dstride=None):

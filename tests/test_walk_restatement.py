@@ -1,23 +1,23 @@
 """The sorted-list KSG algorithm restated on the CPU, against the oracle.
 
 netsci_amd/csrc/ksg_walk.h replaces the all-frames k-NN search with a walk
-over per-node lists of frames sorted by (marginal distance, frame) and the
-marginal counts with upper-bound rank lookups. Its exactness rests on three
-facts: a frame already walked in the other list is recognised by a
-lexicographic (distance, frame) compare; once max(last walked keys) exceeds
-the (k+1)-th smallest joint distance no unwalked frame can enter the k+1
-nearest; and n_x + 1 = rank_a[i][j*] for the selected frame j* of largest
-marginal distance. The kernel keeps its k+1 nearest as 32-bit entries
-(distance code << id_bits | frame), the code a monotone, lossy function of
-the distance's float bits in a window below the pair's coordinate span;
-walks whose k-th and (k+1)-th codes are equal go to the tie resolution,
-which separates code collisions from exact ties. This test runs the same
-steps in numpy (one list block of 8 entries at a time, chosen by the
-kernel's remaining-entries estimate) and requires the oracle's per-frame
-counts (mutual_information_cpu.cpp:22-109), ties and duplicate frames
-included, with the kernel's code step and with a far coarser one that makes
-most walks collide. It pins the algorithm, not the kernel;
-tests/test_gpu_ksg.py pins the kernel.
+over one per-node list of frames sorted by (marginal distance, frame) and
+the marginal counts with upper-bound rank lookups. Its exactness rests on
+three facts: the joint distance is at least either marginal distance, so
+once the walked list's last key exceeds the (k+1)-th smallest joint
+distance no unwalked frame can enter the k+1 nearest (whichever list is
+walked); the query frame itself is skipped; and n_x + 1 = rank_a[i][j*] for
+the selected frame j* of largest marginal distance. The kernel keeps its k+1
+nearest as 32-bit entries (distance code << id_bits | frame), the code a
+monotone, lossy function of the distance's float bits in a window below the
+pair's coordinate span; walks whose k-th and (k+1)-th codes are equal go to
+the tie resolution, which separates code collisions from exact ties. This
+test runs the same steps in numpy (one list block of 8 entries at a time,
+the list chosen by the kernel's prior, or forced to either node) and
+requires the oracle's per-frame counts (mutual_information_cpu.cpp:22-109),
+ties and duplicate frames included, with the kernel's code step and with a
+far coarser one that makes most walks collide. It pins the algorithm, not
+the kernel; tests/test_gpu_ksg.py pins the kernel.
 """
 import numpy as np
 import pytest
@@ -56,7 +56,13 @@ def _id_bits(n):
     return b
 
 
-def _walk_counts(xa, xb, n, d, k, bug=None, shift=None):
+def _prior_pos(n):
+    # ksg_walk.h prior_pos(n, build_width(n)): 64, or for rows of at most 64
+    # frames the last multiple of 8 below n.
+    return 64 if n > 64 else ((n - 1) // 8) * 8
+
+
+def _walk_counts(xa, xb, n, d, k, bug=None, shift=None, choice="prior"):
     ca, cb = xa.reshape(d, n), xb.reshape(d, n)
     oa, ra = _tables(ca, n)
     ob, rb = _tables(cb, n)
@@ -68,6 +74,7 @@ def _walk_counts(xa, xb, n, d, k, bug=None, shift=None):
     base = max(_bits(span) - (window - 1), 0)
     mask = (1 << idb) - 1
     sentinel = 0xFFFFFFFF
+    pp = _prior_pos(n)
 
     def code(key):
         return max(_bits(key) - base, 0) >> shift
@@ -76,47 +83,27 @@ def _walk_counts(xa, xb, n, d, k, bug=None, shift=None):
     ny = np.zeros(n, np.int64)
     for i in range(n):
         DA, DB = _marg(ca, i), _marg(cb, i)
+        # The list: node a's when its key at the prior position is at least
+        # node b's (the kernel's choice), or forced.
+        if choice == "prior":
+            use_a = DA[oa[i, pp]] >= DB[ob[i, pp]]
+        else:
+            use_a = choice == "a"
+        order, DL, DO = (oa, DA, DB) if use_a else (ob, DB, DA)
         h = [sentinel] * (k + 1)
-        last = {"a": (-1.0, -1), "b": (-1.0, -1)}  # (key, frame); -1: none yet
-        pos = {"a": 0, "b": 0}
-        done = False
-        while not done:
-            # list choice per 8-entry block, as the kernel
-            hk = np.uint32(((h[k] >> idb) << shift) + base).view(np.float32) \
-                if h[k] != sentinel else None
-            if pos["a"] >= n:
-                L = "b"
-            elif pos["b"] >= n:
-                L = "a"
-            elif pos["a"] < 8 or pos["b"] < 8 or hk is None:
-                L = "a" if pos["a"] <= pos["b"] else "b"
-            else:
-                with np.errstate(over="ignore"):
-                    ea = pos["a"] * ((hk / max(last["a"][0], 1e-30)) ** d - 1)
-                    eb = pos["b"] * ((hk / max(last["b"][0], 1e-30)) ** d - 1)
-                L = "a" if ea <= eb else "b"
-            O = "b" if L == "a" else "a"
-            order = oa if L == "a" else ob
-            for _ in range(8):
-                if pos[L] >= n:
-                    break
-                j = order[i, pos[L]]
-                pos[L] += 1
-                kL, kO = (DA[j], DB[j]) if L == "a" else (DB[j], DA[j])
-                seen = (kO, j) <= last[O] if last[O][1] >= 0 else False
-                if bug == "no_seen":
-                    seen = False
-                elif bug == "key_only_seen":
-                    seen = kO <= last[O][0] if last[O][1] >= 0 else False
-                last[L] = (kL, j)
-                if not seen and j != i:
+        pos = 0
+        while True:
+            blk = order[i, pos:pos + 8]
+            for j in blk:
+                if j != i or bug == "self_inserted":
                     c = (code(max(DA[j], DB[j])) << idb) | int(j)
                     h = sorted(h + [c])[:k + 1]
-            # the stopping rule once per block, as the kernel; a list walked
-            # to its end has evaluated every frame
-            keys = [v[0] for v in last.values() if v[1] >= 0]
-            if pos[L] >= n or (code(max(keys)) << idb) > h[k]:
-                done = True
+            # the stopping rule once per block, on the block's last key, as
+            # the kernel; a list walked to its end has evaluated every frame
+            key7 = (DO if bug == "stop_on_other_key" else DL)[blk[-1]]
+            pos += len(blk)
+            if pos >= n or (code(key7) << idb) > h[k]:
+                break
         if (h[k - 1] ^ h[k]) >> idb == 0 and bug != "collisions_as_distinct":
             # Tie resolution: exact k-th / (k+1)-th distances over the other
             # frames; equal ones take the std::sort order.
@@ -158,15 +145,17 @@ def _case(kind, d, n, rng):
 @pytest.mark.parametrize("d", [1, 2, 3])
 def test_walk_counts_equal_oracle(kind, d, coarse):
     rng = np.random.default_rng(hash((kind, d)) % 2**32)
-    for _ in range(3):
+    for t in range(3):
         n = int(rng.integers(9, 90))
         k = int(rng.integers(1, min(8, n - 1) + 1))
         xa, xb = _case(kind, d, n, rng)
-        # coarse: 2^19 float-bit units per code step, 1/16 binade
-        nx, ny = _walk_counts(xa, xb, n, d, k, shift=19 if coarse else None)
         ox, oy, _, _ = oracle.ksg_counts(xa, xb, k, n, d)
-        np.testing.assert_array_equal(nx, ox)
-        np.testing.assert_array_equal(ny, oy)
+        # the kernel's choice, and either list forced: any list is exact
+        for choice in ("prior", "a", "b") if t == 0 else ("prior",):
+            # coarse: 2^19 float-bit units per code step, 1/16 binade
+            nx, ny = _walk_counts(xa, xb, n, d, k, shift=19 if coarse else None, choice=choice)
+            np.testing.assert_array_equal(nx, ox, err_msg=choice)
+            np.testing.assert_array_equal(ny, oy, err_msg=choice)
 
 
 def test_code_window_holds_every_key():
@@ -186,15 +175,15 @@ def test_code_window_holds_every_key():
             assert codes.max() < (1 << (32 - idb))
 
 
-@pytest.mark.parametrize("bug,kind", [("no_seen", "dups"), ("key_only_seen", "grid"),
+@pytest.mark.parametrize("bug,kind", [("self_inserted", "gauss"), ("stop_on_other_key", "mixed"),
                                       ("collisions_as_distinct", "gauss")])
 def test_planted_mistakes_are_caught(bug, kind):
-    """A restatement that inserts a frame walked in both lists twice (no
-    seen test), treats every frame whose key equals the other list's last
-    key as walked (no frame tie-break), or takes the k entries of a walk
-    whose k-th and (k+1)-th codes collide as the k nearest, must disagree
-    with the oracle on duplicate-frame, tie-heavy or coarse-code inputs, so
-    the comparison above can fail."""
+    """A restatement that inserts the query frame itself, stops on the
+    other node's distance of the block's last entry instead of the walked
+    list's key, or takes the k entries of a walk whose k-th and (k+1)-th
+    codes collide as the k nearest, must disagree with the oracle on
+    ordinary, mixed-scale or coarse-code inputs, so the comparison above can
+    fail."""
     rng = np.random.default_rng(11)
     mismatched = 0
     for _ in range(4):
