@@ -1057,7 +1057,12 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     unsigned h[K + 1];
     // List blocks: cur holds entries [pos, pos + 8), nxt the 8 after them
     // (pos is a multiple of 8 at every body start until the list ends).
+    // nxt is requested at the end of the body before the one that reads
+    // it, so its load has a whole body to arrive.
     uint4 cur = {0, 0, 0, 0}, nxt = {0, 0, 0, 0};
+    // The lane's list at entry pos (a block of its list, also once the
+    // walk has ended, so that every block request stays in its row).
+    const uint16_t *lpos = a.ord;
     // Counts of the previous refill round, stored one round later so that
     // the rank lookups' latency is not waited for.
     bool pend = false;
@@ -1194,8 +1199,10 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 off = useA ? offA : offB;
                 offO = useA ? offB : offA;
                 base = (unsigned)meta[3 * b + 2];
-                cur = load_block(a.ord + off);
-                if (rs > 8) nxt = load_block(a.ord + table_at(off, 8));
+                // Entries [0, 8) and [8, 16) (table_at(off, 8) = off + 64).
+                lpos = a.ord + off;
+                cur = load_block(lpos);
+                nxt = load_block(rs > 8 ? lpos + 64 : lpos);
                 state = 1;
             }
         }
@@ -1203,10 +1210,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         // next pass keeps the loop's registers in place), and the next
         // pass is a checkpoint.
         if (__ballot(state == 1) == 0) pass = (pass / a.checkpoint + 1) * a.checkpoint - 1;
-        // A body: the next 8 entries of the lane's list. The block two ahead
-        // is requested now and used one body later.
-        uint4 tmp = {0, 0, 0, 0};  // unused when the list ends before that block
-        if (state == 1 && pos + 16 < rs) tmp = load_block(a.ord + table_at(off, pos + 16));
+        // A body: the next 8 entries of the lane's list.
         const bool walks = state == 1;
         const int rem = n - pos;  // entries left in the list
         float key7 = 0.0f;        // the list's key of the body's last entry
@@ -1239,8 +1243,19 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         const bool fin = walks & ((rem <= 8) | (stop > h[K]));
         res = res | fin;
         state = fin ? 0 : state;
+        // The next body's block, and the request of the one after it
+        // (table_at(off, pos + 8) = lpos + 64), by every lane: a lane that
+        // will not walk, or whose list ends before that block, reads its
+        // current block again.
+        const bool go = walks & !fin;
+        lpos += go ? 64 : 0;
+        const uint16_t *lnext = go && pos + 8 < rs ? lpos + 64 : lpos;
         cur = nxt;
-        nxt = tmp;
+        // The copy is made before the request is issued, so the request can
+        // land in nxt's own registers: issued first, it would land in
+        // temporaries whose copy into nxt waits for it at once.
+        asm volatile("" : "+v"(cur.x), "+v"(cur.y), "+v"(cur.z), "+v"(cur.w)::"memory");
+        nxt = load_block(lnext);
     }
     if (pend) __builtin_nontemporal_store((pend_x - 1u) | ((pend_y - 1u) << 16), a.cnt + pend_at);
     // Per-wave totals.
