@@ -183,6 +183,14 @@ int nsx_ksg_last_list_info(int *blocks, long long *slots, long long *node_builds
  * applies from the next call. No reference counterpart. */
 int nsx_ksg_set_table_budget(long long bytes);
 
+/* Entries kept per sorted list of the list path's tables (rounded up to 8,
+ * at least 8); 0 = the default, all of a row up to 512 frames and max(512,
+ * n / 8) beyond. Walks that reach the end of a cut list without meeting
+ * their stopping rule are resolved exactly by the tie kernels, so any cap
+ * gives the same results; a smaller one saves memory. Applies from the next
+ * call. No reference counterpart. */
+int nsx_ksg_set_list_cap(long long entries);
+
 /* Frees the library's device workspaces and tables (synchronises the
  * device); the next call allocates what it needs again. No reference
  * counterpart. */

@@ -48,6 +48,7 @@ SIGNATURES = {
     "nsx_ksg_last_call_ms": [ctypes.POINTER(ctypes.c_double)],
     "nsx_ksg_last_list_info": [_IP, _LLP, _LLP],
     "nsx_ksg_set_table_budget": [_LL],
+    "nsx_ksg_set_list_cap": [_LL],
     "nsx_release_workspace": [],
     "nsx_ksg_plan_create": [ctypes.POINTER(_VP), _VP, _LL, _LL, _I, _IP, _IP, _I, _VP, _LL, _I,
                             _VP, _I],
@@ -278,12 +279,25 @@ def set_table_budget(nbytes=0):
     check(lib().nsx_ksg_set_table_budget(int(nbytes)), "set table budget")
 
 
-def table_slot_bytes(n):
-    """Device bytes of one node's list-path tables for windows of n frames:
-    the ord and rank tables (rows of n rounded up to 8, 2 bytes per entry)
-    and one float prior per row (ksg.hip's slot_bytes_for)."""
+def set_list_cap(entries=0):
+    """Entries kept per sorted list of the list path's tables (0 = default:
+    whole rows up to 512 frames, max(512, n / 8) beyond). Any cap gives the
+    same results: walks that reach the end of a cut list go to the tie
+    kernels."""
+    check(lib().nsx_ksg_set_list_cap(int(entries)), "set list cap")
+
+
+def table_slot_bytes(n, cap=0):
+    """Device bytes of one node's list-path tables for windows of n frames
+    (ksg.hip's slot_bytes_for) under list cap `cap` (0 = default): rows of
+    n rounded up to 8, each with the first entries of its sorted list and a
+    rank per frame (2 bytes each), and one float prior per row."""
     rows = (n + 7) // 8 * 8
-    return 2 * rows * rows * 2 + rows * 4
+    if cap > 0:
+        ors = min(rows, max(8, (cap + 7) // 8 * 8))
+    else:
+        ors = min(rows, max(512, (n // 8 + 7) // 8 * 8))
+    return rows * (ors + rows) * 2 + rows * 4
 
 
 def release_workspace():

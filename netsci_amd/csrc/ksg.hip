@@ -607,14 +607,25 @@ int walk_shape(const Problem &pb, WalkShape &ws) {
     return kOk;
 }
 
-// Bytes of one node slot's ord (or rank) table, and of the whole slot (ord,
-// rank and the per-frame priors), for windows of up to nmax frames.
-size_t table_bytes_for(int nmax) {
-    const long long rs = ((long long)nmax + 7) & ~7LL;
-    return (size_t)ksgw::table_rows(nmax) * (size_t)rs * sizeof(uint16_t);
+// Entries kept per ord row for windows of n frames: ksgw::ord_stride(n), or
+// the cap nsx_ksg_set_list_cap set (rounded up to 8, at most the row).
+long long g_list_cap = 0;
+long long list_stride(int n) {
+    const long long rs = ((long long)n + 7) & ~7LL;
+    if (g_list_cap <= 0) return ksgw::ord_stride(n);
+    return std::min(rs, std::max<long long>(8, (g_list_cap + 7) & ~7LL));
+}
+
+// Entries of one node slot's ord and rank tables, and bytes of the whole
+// slot (ord, rank and the per-frame priors), for windows of up to nmax
+// frames (a shorter window's slots are smaller).
+long long ord_entries_for(int n) { return ksgw::table_rows(n) * list_stride(n); }
+long long rank_entries_for(int n) {
+    return ksgw::table_rows(n) * (((long long)n + 7) & ~7LL);
 }
 size_t slot_bytes_for(int nmax) {
-    return 2 * table_bytes_for(nmax) + (size_t)ksgw::table_rows(nmax) * sizeof(float);
+    return (size_t)(ord_entries_for(nmax) + rank_entries_for(nmax)) * sizeof(uint16_t) +
+           (size_t)ksgw::table_rows(nmax) * sizeof(float);
 }
 
 // The schedule for U used nodes in S slots (see above). sched_of[x * nb + y]
@@ -786,10 +797,9 @@ int list_prepare(const Problem &pb, ListLayout &L, hipStream_t stream, bool *ok,
                         "ksg: no device memory for two nodes' list-path tables");
             return kOk;
         }
-        const long long tab_entries = (long long)(table_bytes_for(pb.nmax) / sizeof(uint16_t));
         L.ord = (uint16_t *)tab;
-        L.rank = L.ord + L.S * tab_entries;
-        L.prior = reinterpret_cast<float *>(L.rank + L.S * tab_entries);
+        L.rank = L.ord + L.S * ord_entries_for(pb.nmax);
+        L.prior = reinterpret_cast<float *>(L.rank + L.S * rank_entries_for(pb.nmax));
         make_schedule(L, sched_of);
     }
 
@@ -915,7 +925,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
         const int off = pb.host_win[w], n = pb.host_win[pb.R + w], rep = pb.host_win[2 * pb.R + w];
         const float *X = pb.X + off;
         const long long rs = ((long long)n + 7) & ~7LL;
-        const long long slot_entries = ksgw::table_rows(n) * rs;
+        const long long ors = list_stride(n);
         float *out = mi_out + (long long)rep * pb.P;
         const float *span = L.span + (long long)w * L.N;
         const int width = ksgw::build_width(n);
@@ -953,6 +963,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 ba.dstride = pb.dstride;
                 ba.n = n;
                 ba.rs = rs;
+                ba.ors = ors;
                 ba.node_of = L.node_of + first;
                 const int block_waves = ksgw::build_block_waves(width);
                 const int rows_per_pass = block_waves / ksgw::build_row_waves(width);
@@ -970,8 +981,8 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                                 : std::max<long long>(1, std::min<long long>(
                                       ((long long)count * n + rows_per_pass - 1) / rows_per_pass,
                                       64LL * cu_count()));
-                ba.ord = L.ord + (long long)base * slot_entries;
-                ba.rank = L.rank + (long long)base * slot_entries;
+                ba.ord = L.ord + (long long)base * ord_entries_for(n);
+                ba.rank = L.rank + (long long)base * rank_entries_for(n);
                 ba.prior = L.prior + (long long)base * ksgw::table_rows(n);
                 ba.span = span;
                 ba.abort = abort_flag;
@@ -1000,6 +1011,7 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 wa.rank = L.rank;
                 wa.prior = L.prior;
                 wa.rs = rs;
+                wa.ors = ors;
                 wa.slot_of = L.nb > 1 ? L.rslot_of : L.slot_of;
                 wa.cnt = L.cnt;
                 wa.stats = stats;
@@ -1505,6 +1517,15 @@ int nsx_ksg_mi_chunked(const float *d_X, long long ld, long long dstride, int d,
 
 int nsx_ksg_last_tie_frames(long long *count) {
     if (count) *count = g_last_ties;
+    return kOk;
+}
+
+int nsx_ksg_set_list_cap(long long entries) {
+    clear_error();
+    NSX_REQUIRE(entries >= 0 && entries <= ksgw::kMaxFrames,
+                "nsx_ksg_set_list_cap: entries must be 0..8192");
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_list_cap = entries;
     return kOk;
 }
 

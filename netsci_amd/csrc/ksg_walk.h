@@ -79,7 +79,8 @@ struct BuildArgs {
     long long ld;       // row stride of X (floats)
     long long dstride;  // stride between coordinate dimensions (floats)
     int n;              // frames
-    long long rs;       // row stride of the tables (entries, a multiple of 8)
+    long long rs;       // row stride of the rank tables (entries, a multiple of 8)
+    long long ors;      // row stride of the ord tables: the entries kept, ord_stride(n)
     const int *node_of; // table slot -> node
     int groups;         // workgroups per node (rows of <= 2048 frames)
     int nodes;          // nodes (table slots) of the launch
@@ -105,7 +106,8 @@ struct WalkArgs {
     long long P;   // pairs of this launch
     int n, k;
     const uint16_t *ord, *rank;
-    long long rs;
+    long long rs;               // rank row stride
+    long long ors;              // ord row stride (entries kept per list, ord_stride(n))
     const int *slot_of;         // node -> table slot
     unsigned *cnt;              // [P][n] packed counts n_x | n_y << 16
     unsigned long long *stats;  // [0] tie frames, [1] list entries walked (both lists)
@@ -187,6 +189,19 @@ NSX_HD inline long long table_row(int slot, int i, int n, long long rs) {
 }
 NSX_HD inline long long table_at(long long row, long long e) {
     return row + (e >> 3) * (8 * kRowGroup) + (e & 7);
+}
+
+// Entries kept per ord row. A walk reads a prefix of its list (on C-alpha
+// trajectories the longest walks are a few hundred entries at n = 5000), so
+// rows of more than 512 frames keep max(512, n / 8) entries, rounded up to
+// 8; a walk that reaches the end of a cut row without meeting the stopping
+// rule goes to the tie kernels, which work on the whole row. The rank
+// tables keep every frame (a count can reach far beyond the walk).
+NSX_HD inline long long ord_stride(int n) {
+    const long long rs = ((long long)n + 7) & ~7LL;
+    long long c = ((long long)n / 8 + 7) & ~7LL;
+    c = c < 512 ? 512 : c;
+    return rs < c ? rs : c;
 }
 
 // The list position whose marginal distance (the "prior" of a node's frame)
@@ -537,12 +552,13 @@ __global__ __launch_bounds__(kWave *kBuildWaves, 3) void list_build_kernel(Build
         }
         if (lane == prior_pos(n, R) / R) a.prior[(long long)slot * table_rows(n) + i] = kx[0];
 
-        // ord row: this lane's R consecutive positions, 16 bytes at a time.
-        const long long orow = table_row(slot, i, n, a.rs);
+        // ord row (its first a.ors positions): this lane's R consecutive
+        // positions, 16 bytes at a time.
+        const long long orow = table_row(slot, i, n, a.ors);
 #pragma unroll
         for (int r0 = 0; r0 < R; r0 += 8) {
             const long long e0 = (long long)lane * R + r0;
-            if (e0 < a.rs) {
+            if (e0 < a.ors) {
                 // Padding positions (past n) hold the row's own frame i: the
                 // walk reads whole blocks and never inserts frame i.
                 unsigned e8[8];
@@ -804,12 +820,12 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
         if (row && wl == 0 && lane == kPriorPos / 32)
             a.prior[(long long)slot * table_rows(n) + i] = __uint_as_float(((u[0] >> 13) << 8) + cbase);
 
-        // ord row.
-        const long long orow = table_row(slot, row ? i : 0, n, a.rs);
+        // ord row (its first a.ors positions).
+        const long long orow = table_row(slot, row ? i : 0, n, a.ors);
 #pragma unroll
         for (int r0 = 0; r0 < 32; r0 += 8) {
             const long long e0 = (long long)ebase + lane * 32 + r0;
-            if (row && e0 < a.rs) {
+            if (row && e0 < a.ors) {
                 // Padding positions (past n) hold the row's own frame i: the
                 // walk reads whole blocks and never inserts frame i.
                 unsigned e8[8];
@@ -1017,7 +1033,10 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     const int idb = a.id_bits, shift = a.code_shift;
     const unsigned id_mask = (1u << idb) - 1u;
     const int lane = threadIdx.x & (kWave - 1);
-    const long long rs = a.rs;
+    const long long rs = a.rs, ors = a.ors;
+    // Entries of a list: n, or the ord rows' length when they are cut.
+    const int nl = (long long)n < ors ? n : (int)ors;
+    const bool cut_rows = nl < n;
     const long long G = gridDim.x;
     const long long npairs =
         (long long)blockIdx.x < a.P ? (a.P - (long long)blockIdx.x + G - 1) / G : 0;
@@ -1038,14 +1057,14 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
     // not staged yet, 1 walks, 2 no queries left.
     int state = 0;
     bool res = false;  // a finished walk waits for its counts
+    bool inc = false;  // the walk reached the end of a cut row without stopping
     int q = 0;   // the lane's query number: frame i = q - qs * n of local pair qs
     int qs = 0;
     int i = 0, pos = 0;  // entries walked in the lane's list
     int rec = 0;  // LDS float offset of the lane's pair records (an offset, not a
                   // pointer, so every access stays an LDS instruction)
-    // Table row offsets of (node, frame i): the walked list's and the other
-    // node's (its ranks are looked up at the end).
-    long long off = 0, offO = 0;
+    // Rank table row offsets of (node a, frame i) and (node b, frame i).
+    long long rA = 0, rB = 0;
     bool useA = true;  // the walked list is node a's
     unsigned base = 0; // the pair's code base
     float qa[D], qb[D];
@@ -1114,8 +1133,10 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
             if (pend)
                 __builtin_nontemporal_store((pend_x - 1u) | ((pend_y - 1u) << 16), a.cnt + pend_at);
             pend = false;
-            // (1) eps of the finished walks.
-            const bool tie = res && ((h[K - 1] ^ h[K]) >> idb) == 0u;
+            // (1) eps of the finished walks. Walks that ended at the end of a
+            // cut row without meeting the stopping rule go to the tie list
+            // with the ties (the tie kernels work on the whole row).
+            const bool tie = res && (inc || ((h[K - 1] ^ h[K]) >> idb) == 0u);
             int jx = 0, jy = 0;
             if (res && !tie) {
                 float ex = -1.0f, ey = -1.0f;
@@ -1155,9 +1176,8 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                     // (0 - 1) | (0 - 1) << 16 is kTieMark.
                     // Non-temporal: each rank line serves one lookup, and
                     // keeping it out of L2 leaves room for the list rows.
-                    const long long offA = useA ? off : offO, offB = useA ? offO : off;
-                    pend_x = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(offA, jx));
-                    pend_y = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(offB, jy));
+                    pend_x = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(rA, jx));
+                    pend_y = tie ? 0u : (unsigned)__builtin_nontemporal_load(a.rank + table_at(rB, jy));
                     pend_at = ((long long)blockIdx.x + qs * G) * n + i;
                     pend = true;
                 }
@@ -1165,6 +1185,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 atomicAdd(done + ring_slot(qs, nbuf), 1);  // its LDS records are no longer read
             }
             res = false;
+            inc = false;
             // (2) reserve query numbers for the lanes that want one.
             if (want) {
                 const int leader = __builtin_amdgcn_readfirstlane(__ffsll((long long)want) - 1);
@@ -1194,15 +1215,14 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
                 for (int m = 0; m <= K; ++m) h[m] = 0xffffffffu;
                 pos = 0;
                 useA = ((choice[b * cwords + (i >> 5)] >> (i & 31)) & 1u) != 0u;
-                const long long offA = table_row(meta[3 * b], i, n, rs);
-                const long long offB = table_row(meta[3 * b + 1], i, n, rs);
-                off = useA ? offA : offB;
-                offO = useA ? offB : offA;
+                rA = table_row(meta[3 * b], i, n, rs);
+                rB = table_row(meta[3 * b + 1], i, n, rs);
                 base = (unsigned)meta[3 * b + 2];
-                // Entries [0, 8) and [8, 16) (table_at(off, 8) = off + 64).
-                lpos = a.ord + off;
+                // Entries [0, 8) and [8, 16) of the list (table_at(row, 8) =
+                // row + 64).
+                lpos = a.ord + table_row(meta[3 * b + (useA ? 0 : 1)], i, n, ors);
                 cur = load_block(lpos);
-                nxt = load_block(rs > 8 ? lpos + 64 : lpos);
+                nxt = load_block(ors > 8 ? lpos + 64 : lpos);
                 state = 1;
             }
         }
@@ -1212,7 +1232,7 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         if (__ballot(state == 1) == 0) pass = (pass / a.checkpoint + 1) * a.checkpoint - 1;
         // A body: the next 8 entries of the lane's list.
         const bool walks = state == 1;
-        const int rem = n - pos;  // entries left in the list
+        const int rem = nl - pos;  // entries left in the list
         float key7 = 0.0f;        // the list's key of the body's last entry
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
@@ -1237,19 +1257,22 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         // The rule is monotone (keys only grow, h[K] only falls), so a walk
         // that meets it inside the body meets it here too, and the entries
         // after that point change nothing. A list walked to its end has
-        // evaluated every frame.
+        // evaluated every frame; a cut row walked to its end has not, and
+        // the walk is marked incomplete.
         pos += walks ? min(rem, 8) : 0;
         const unsigned stop = key_code(key7, base, shift) << idb;
-        const bool fin = walks & ((rem <= 8) | (stop > h[K]));
+        const bool met = stop > h[K];
+        const bool fin = walks & ((rem <= 8) | met);
         res = res | fin;
+        inc = inc | (fin & !met & cut_rows);
         state = fin ? 0 : state;
         // The next body's block, and the request of the one after it
-        // (table_at(off, pos + 8) = lpos + 64), by every lane: a lane that
+        // (table_at(row, pos + 8) = lpos + 64), by every lane: a lane that
         // will not walk, or whose list ends before that block, reads its
         // current block again.
         const bool go = walks & !fin;
         lpos += go ? 64 : 0;
-        const uint16_t *lnext = go && pos + 8 < rs ? lpos + 64 : lpos;
+        const uint16_t *lnext = go && pos + 8 < ors ? lpos + 64 : lpos;
         cur = nxt;
         // The copy is made before the request is issued, so the request can
         // land in nxt's own registers: issued first, it would land in

@@ -1,5 +1,6 @@
-"""The list path when its tables do not all fit, and walks whose workgroups
-retire and restage pairs through their LDS ring (GPU required).
+"""The list path when its tables do not all fit, with sorted lists cut
+short, and walks whose workgroups retire and restage pairs through their
+LDS ring (GPU required).
 
 Every case compares all pairs bitwise with the all-frames kernel (an
 independent algorithm over every frame pair) and with the oracle
@@ -23,9 +24,11 @@ def _flush(msg):
 def _reset():
     abi.set_method(abi.METHOD_AUTO)
     abi.set_table_budget(0)
+    abi.set_list_cap(0)
     yield
     abi.set_method(abi.METHOD_AUTO)
     abi.set_table_budget(0)
+    abi.set_list_cap(0)
 
 
 def _slot_bytes(n):
@@ -200,3 +203,33 @@ def test_tables_persist_between_calls():
     third, _, _, _ = _mi(X, ab, k, n, d, abi.METHOD_LISTS)
     np.testing.assert_array_equal(first, third)
     assert held1 == N and held2 <= N
+
+
+@pytest.mark.parametrize("n,d,k,cap", [(300, 3, 6, 8), (300, 3, 6, 24), (1000, 2, 4, 16),
+                                       (2600, 3, 6, 64), (700, 1, 6, 40)])
+def test_cut_lists_resolved_exactly(n, d, k, cap):
+    """Sorted lists cut to a few entries: most walks reach the end of their
+    cut list before the stopping rule holds and go to the tie kernels, which
+    resolve them on the whole row; every pair still equals the default
+    tables' result, the all-frames kernel and the oracle, with and without
+    per-frame counts."""
+    N = 9
+    _flush(f"cut lists n={n} d={d} k={k} cap={cap}")
+    X, _ = synth.calpha_trajectory(N=N, n=n, seed=70 + cap)
+    if d < 3:
+        X = np.ascontiguousarray(X[:, :d * n])
+    ab = np.concatenate([synth.all_pairs(N), np.array([[2, 2], [8, 1]], np.int32)])
+    want, wcnt, m0, _ = _mi(X, ab, k, n, d, abi.METHOD_LISTS, counts=True)
+    ties0 = abi.last_tie_frames()
+    assert m0 == 2
+    abi.set_list_cap(cap)
+    got, gcnt, m1, _ = _mi(X, ab, k, n, d, abi.METHOD_LISTS, counts=True)
+    ties1 = abi.last_tie_frames()
+    assert m1 == 2
+    assert ties1 > ties0  # walks cut short went to the tie kernels
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(gcnt, wcnt)
+    af, _, m2, _ = _mi(X, ab, k, n, d, abi.METHOD_ALL_FRAMES)
+    assert m2 == 1
+    np.testing.assert_array_equal(got, af)
+    _oracle_sample(X, ab, got, k, n, d, 8, seed=cap)
