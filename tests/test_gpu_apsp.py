@@ -138,6 +138,32 @@ def test_sparse_steps_long_lists(L):
     np.testing.assert_array_equal(dHi.numpy().view(np.uint32), Hio.view(np.uint32))
 
 
+def test_sparse_steps_mixed_tiles():
+    """A banded graph whose first 64 rows are dense-ish: the first j tile's
+    neighbour union exceeds the LDS staging capacity (its outputs read H_p
+    per list entry), the others fold from their staged unions; lengths and
+    snapshots must equal the oracle."""
+    N, L = 700, 5
+    print(f"apsp mixed tiles N={N} L={L}", flush=True)
+    rng = np.random.default_rng(77)
+    A = rng.uniform(0.1, 2.0, (N, N)).astype(np.float32)
+    band = np.abs(np.arange(N)[:, None] - np.arange(N)[None, :]) <= 5
+    keep = band.copy()
+    keep[:64] |= rng.random((64, N)) < 0.3
+    A[~keep] = np.inf
+    np.fill_diagonal(A, 0.0)
+    assert ((A < np.inf).sum(axis=1) + 3) // 4 * 4 @ np.ones(N) / N < 128  # sparse path
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    Ho, Hio, _ = oracle.hedetniemi(A, L, 1e-4, want_hi=True, want_paths=False)
+    abi.hedetniemi_lengths(dA, N, L, dH)
+    np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
+    abi.hedetniemi(dA, N, L, 1e-4, dH, dHi, None)
+    np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
+    np.testing.assert_array_equal(dHi.numpy().view(np.uint32), Hio.view(np.uint32))
+
+
 def test_lengths_only_matches():
     N, L = 150, 9
     print("apsp lengths", flush=True)
