@@ -126,9 +126,20 @@ def main():
         print(json.dumps({"copy_bytes": nbytes}), flush=True)
         return
     if args.valu_probe:
+        # The probe is a separate measurement library (libnsx_probe.so),
+        # not part of libnsx's C ABI.
+        import ctypes
+        probe = ctypes.CDLL(os.path.join(ROOT, "netsci_amd", "libnsx_probe.so"))
+        probe.nsx_valu_probe.argtypes = [ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_longlong)]
         res = {}
         for kind, name in enumerate(["v_sub_f32", "v_max3_f32+v_sub_f32", "v_fma_f32", "ksg_mix"]):
-            ms, lanes = abi.probe_valu(kind)
+            ms, lanes = ctypes.c_double(0), ctypes.c_longlong(0)
+            rc = probe.nsx_valu_probe(kind, 20000, ctypes.byref(ms), ctypes.byref(lanes))
+            if rc:
+                raise RuntimeError(f"nsx_valu_probe failed: {rc}")
+            ms, lanes = ms.value, lanes.value
             res[name] = {"ms": ms, "lanes": lanes}
             log(f"probe {name}: {ms:.2f} ms")
         print(json.dumps({"valu_probe": res, "iters": 20000, "ops_per_iter": 32}), flush=True)

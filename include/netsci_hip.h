@@ -53,12 +53,6 @@ int nsx_memcpy_d2d(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int nsx_stream_sync(void *stream);
 int nsx_device_sync(void);
 
-/* VALU throughput probe (a diagnostic for roofline accounting): times one
- * launch of 32 waves per CU running iters x 32 dependent-chain operations
- * per lane of one instruction kind (0 v_sub_f32, 1 v_max3_f32, 2 v_fma_f32,
- * 3 the KSG loop mix). Reports the launch time and the lane count. */
-int nsx_probe_valu(int kind, int iters, double *ms, long long *lanes);
-
 /* Device timing on `stream` (hipEvent pairs). */
 int nsx_event_create(void **ev);
 int nsx_event_destroy(void *ev);

@@ -30,7 +30,6 @@ SIGNATURES = {
     "nsx_memcpy_d2d": [_VP, _VP, _SZ, _VP],
     "nsx_stream_sync": [_VP],
     "nsx_device_sync": [],
-    "nsx_probe_valu": [_I, _I, ctypes.POINTER(ctypes.c_double), _LLP],
     "nsx_event_create": [ctypes.POINTER(_VP)],
     "nsx_event_destroy": [_VP],
     "nsx_event_record": [_VP, _VP],
@@ -382,13 +381,6 @@ def hedetniemi_lengths(dA, N, L, dH, stream=None):
 
 def corr_to_adj(dC, N, dA, stream=None):
     check(lib().nsx_corr_to_adj(dC.ptr, N, dA.ptr, stream), "nsx_corr_to_adj")
-
-
-def probe_valu(kind, iters=20000):
-    """(ms, lanes) of one nsx_probe_valu launch."""
-    ms, lanes = ctypes.c_double(0), ctypes.c_longlong(0)
-    check(lib().nsx_probe_valu(kind, iters, ctypes.byref(ms), ctypes.byref(lanes)), "probe")
-    return ms.value, lanes.value
 
 
 def sync():

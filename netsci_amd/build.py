@@ -4,6 +4,8 @@
   include/netsci_hip.h and the host (CPU platform) implementations.
 * ``_netsci*.so`` — g++: the C++ CuArray / netcalc host layer and its
   pybind11 bindings, linked against libnsx.so.
+* ``libnsx_probe.so`` — hipcc: the FP32 VALU issue-rate probe that
+  bench.py --valu-probe times (a measurement tool, not part of the C ABI).
 
 Run ``python -m netsci_amd.build`` (or ``__graft_entry__.build()``).
 """
@@ -51,6 +53,10 @@ def lib_path():
     return os.path.join(HERE, "libnsx.so")
 
 
+def probe_path():
+    return os.path.join(HERE, "libnsx_probe.so")
+
+
 def ext_path():
     suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     return os.path.join(HERE, "_netsci" + suffix)
@@ -79,6 +85,9 @@ def build(verbose=False):
          ["-pthread", "-L/opt/rocm/lib", "-lrocblas", "-lrocprofiler-sdk-roctx",
           "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, lib_path())
+    tmp = probe_path() + ".tmp"
+    _run([hipcc] + HIPCC_FLAGS + ["-shared", os.path.join(CSRC, "valu_probe.hip"), "-o", tmp])
+    os.replace(tmp, probe_path())
 
     import pybind11
     inc = [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}"]

@@ -148,49 +148,6 @@ int fill_u32(void *d_ptr, uint32_t value, size_t count, hipStream_t stream) {
     return kOk;
 }
 
-// VALU throughput probe: 8 independent dependency chains per lane of the
-// instruction mix under test; every result feeds the output so nothing is
-// folded away. kind 0: v_sub_f32; 1: v_max3_f32 (|.| modifiers);
-// 2: v_fma_f32; 3: the KSG inner-loop mix (sub, max3, compare, select).
-__global__ __launch_bounds__(256) void valu_probe_kernel(float *out, float a, float b, int iters,
-                                                         int kind) {
-    float x[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) x[c] = threadIdx.x * 1e-3f + c;
-    if (kind == 0) {
-        for (int it = 0; it < iters; ++it)
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) x[c] = x[c] - a;
-    } else if (kind == 1) {
-        for (int it = 0; it < iters; ++it)
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) x[c] = fmaxf(fmaxf(fabsf(x[c]), fabsf(a)), fabsf(b - x[c]));
-    } else if (kind == 2) {
-        for (int it = 0; it < iters; ++it)
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) x[c] = fmaf(x[c], a, b);
-    } else {
-        for (int it = 0; it < iters; ++it)
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int c = 0; c < 8; ++c) {
-                    const float d = fmaxf(fabsf(x[c] - a), fabsf(x[c] - b));
-                    x[c] = (d < b) ? d : x[c] - a;
-                }
-    }
-    float s = 0.f;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) s += x[c];
-    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
 void psi_table(float *psi, int n) {
     // psi.cpp:6-19: zero-initialised, psi[1] = (float)-0.57721566490153,
     // psi[i+1] = psi[i] + (float)(1.0 / (float)i) in float32.
@@ -405,34 +362,6 @@ int nsx_event_elapsed_ms(void *start, void *stop, float *ms) {
     NSX_HIP(hipEventSynchronize((hipEvent_t)stop), "hipEventSynchronize");
     NSX_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop),
             "hipEventElapsedTime");
-    return kOk;
-}
-
-int nsx_probe_valu(int kind, int iters, double *ms, long long *lanes) {
-    clear_error();
-    NSX_REQUIRE(kind >= 0 && kind <= 3 && iters >= 1 && ms && lanes,
-                "nsx_probe_valu: invalid arguments");
-    int rc = ensure_device();
-    if (rc) return rc;
-    const int blocks = cu_count() * 8;  // 8 x 256 threads per CU = 32 waves
-    float *out = (float *)workspace(kWsScreen0, (size_t)blocks * 256 * sizeof(float));
-    if (!out) return kHipError;
-    hipEvent_t e0, e1;
-    NSX_HIP(hipEventCreate(&e0), "probe event");
-    NSX_HIP(hipEventCreate(&e1), "probe event");
-    hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, nullptr, out, 1.25f, 0.75f,
-                       iters, kind);
-    NSX_HIP(hipEventRecord(e0, nullptr), "probe record");
-    hipLaunchKernelGGL(valu_probe_kernel, dim3(blocks), dim3(256), 0, nullptr, out, 1.25f, 0.75f,
-                       iters, kind);
-    NSX_HIP(hipEventRecord(e1, nullptr), "probe record");
-    NSX_HIP(hipEventSynchronize(e1), "probe sync");
-    float t = 0;
-    NSX_HIP(hipEventElapsedTime(&t, e0, e1), "probe elapsed");
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms = t;
-    *lanes = (long long)blocks * 256;
     return kOk;
 }
 
