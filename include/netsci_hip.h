@@ -113,7 +113,9 @@ int nsx_ksg_mi_noalloc(const float *d_X, long long ld, long long dstride, int d,
  * with the list path's tables built once. When chunk c is done, its MI
  * values are in d_mi_out and h_mi_out[h_ends[c-1] .. h_ends[c]), and
  * on_chunk(user, h_ends[c]) runs on the calling thread while the device
- * works on the next chunk; a non-zero return stops the call with an error.
+ * works on the next chunks; a non-zero return stops the call with an error.
+ * Consecutive chunks of fewer than 128 pairs per CU in all are computed
+ * in one device batch, and their callbacks run in order when it is done.
  * Returns when every chunk is done. */
 typedef int (*nsx_chunk_fn)(void *user, long long end);
 int nsx_ksg_mi_chunked(const float *d_X, long long ld, long long dstride, int d,

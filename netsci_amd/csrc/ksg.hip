@@ -1205,27 +1205,46 @@ int run_chunked(const Problem &pb, float *mi_out, float *h_mi_out, const long lo
             for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         }
     } clean{cs, {}};
-    clean.ev.assign(nends, nullptr);
-    for (int c = 0; c < nends; ++c)
-        NSX_HIP(hipEventCreateWithFlags(&clean.ev[c], hipEventDisableTiming), "chunk event");
-    // Hands chunk c's values to the host and calls back.
-    auto finish = [&](int c) -> int {
-        const long long b = c ? ends[c - 1] : 0, e = ends[c];
-        NSX_HIP(hipStreamWaitEvent(cs, clean.ev[c], 0), "chunk wait");
-        NSX_HIP(hipMemcpyAsync(h_mi_out + b, mi_out + b, (size_t)(e - b) * sizeof(float),
-                               hipMemcpyDeviceToHost, cs),
-                "chunk copy");
-        NSX_HIP(hipStreamSynchronize(cs), "chunk copy sync");
-        if (on_chunk && on_chunk(user, e) != 0) {
-            (void)hipStreamSynchronize(stream);
-            set_error("ksg: stopped by the chunk callback");
-            return kInternal;
+    // Device batches: consecutive chunks grouped until a batch holds at
+    // least kBatchPairsPerCU pairs per CU (the last may hold fewer). A walk
+    // launch ends with its slowest workgroup and the psi sums of a few
+    // thousand pairs cannot fill the device, so each launch costs about
+    // half a millisecond beyond its work; the chunks' callbacks then run in
+    // order once their batch is done.
+    constexpr long long kBatchPairsPerCU = 128;
+    const long long min_batch = kBatchPairsPerCU * cu_count();
+    std::vector<int> last;  // last chunk of each batch
+    for (int c = 0; c < nends; ++c) {
+        const long long b0 = last.empty() ? 0 : ends[last.back()];
+        if (ends[c] - b0 >= min_batch || c == nends - 1) last.push_back(c);
+    }
+    const int nb = (int)last.size();
+    clean.ev.assign(nb, nullptr);
+    for (int t = 0; t < nb; ++t)
+        NSX_HIP(hipEventCreateWithFlags(&clean.ev[t], hipEventDisableTiming), "chunk event");
+    // Hands batch t's chunks to the host one by one (each copied just
+    // before its callback, so a callback sees no later values) and calls
+    // back.
+    auto finish = [&](int t) -> int {
+        const int c0 = t ? last[t - 1] + 1 : 0;
+        NSX_HIP(hipStreamWaitEvent(cs, clean.ev[t], 0), "chunk wait");
+        for (int c = c0; c <= last[t]; ++c) {
+            const long long b = c ? ends[c - 1] : 0, e = ends[c];
+            NSX_HIP(hipMemcpyAsync(h_mi_out + b, mi_out + b, (size_t)(e - b) * sizeof(float),
+                                   hipMemcpyDeviceToHost, cs),
+                    "chunk copy");
+            NSX_HIP(hipStreamSynchronize(cs), "chunk copy sync");
+            if (on_chunk && on_chunk(user, ends[c]) != 0) {
+                (void)hipStreamSynchronize(stream);
+                set_error("ksg: stopped by the chunk callback");
+                return kInternal;
+            }
         }
         return kOk;
     };
     long long ties = 0;
-    for (int c = 0; c < nends; ++c) {
-        const long long b = c ? ends[c - 1] : 0, e = ends[c];
+    for (int c = 0; c < nb; ++c) {
+        const long long b = c ? ends[last[c - 1]] : 0, e = ends[last[c]];
         if (lists) {
             rc = list_enqueue(pb, L, ws, d_psi, mi_out, nullptr, stream, nullptr, b, e, c == 0);
         } else {
@@ -1244,7 +1263,7 @@ int run_chunked(const Problem &pb, float *mi_out, float *h_mi_out, const long lo
             if (rc) return rc;
         }
     }
-    for (int c = std::max(0, nends - 2); c < nends; ++c) {
+    for (int c = std::max(0, nb - 2); c < nb; ++c) {
         rc = finish(c);
         if (rc) return rc;
     }
