@@ -1234,18 +1234,20 @@ __global__ __launch_bounds__(kWave *walk_max_waves(K)) void walk_kernel(WalkArgs
         const bool walks = state == 1;
         const int rem = nl - pos;  // entries left in the list
         float key7 = 0.0f;        // the list's key of the body's last entry
+        // The body's 8 records are requested at once (their gathers overlap
+        // instead of each step waiting for its own). Block entries are
+        // frames (the tables' padding past a list's end holds the query
+        // frame i, which is never inserted; blocks of no walk are 0 or a
+        // previous walk's), so every read stays within the lane's records.
+        float u[8][D], v[8][D];
+#pragma unroll
+        for (int st = 0; st < 8; ++st) fetch<D>(lds + rec, block_entry(cur, st), u[st], v[st]);
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
             // Every lane runs every step (no branches: the loop's registers
             // stay in place); a lane that is not walking inserts nothing.
-            // Block entries are frames (the tables' padding past a list's
-            // end holds the query frame i, which is never inserted; blocks
-            // of no walk are 0 or a previous walk's), so every read stays
-            // within the lane's records.
             const int j = block_entry(cur, st);
-            float u[D], v[D];
-            fetch<D>(lds + rec, j, u, v);
-            const float da = marg<D>(qa, u), db = marg<D>(qb, v);
+            const float da = marg<D>(qa, u[st]), db = marg<D>(qb, v[st]);
             const unsigned c = (key_code(fmaxf(da, db), base, shift) << idb) | (unsigned)j;
             const bool take = walks & (j != i);
             insert_packed<K>(h, take ? c : 0xffffffffu);
