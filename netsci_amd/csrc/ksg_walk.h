@@ -17,9 +17,9 @@
 //     k+1 nearest lie within either list's walked prefix, so either list
 //     gives the same result; the walk takes the one whose keys grow faster
 //     around the query (the larger "prior": the marginal distance at list
-//     position 64, which the table build records per node and frame), which
-//     is the shorter walk for about 97% of the queries of a C-alpha
-//     trajectory.
+//     position 64, which the table build records per node and frame); on a
+//     C-alpha trajectory (N = 1000, n = 2000) that list needs no more
+//     8-entry blocks than the other for 98% of the queries.
 //
 // The marginal counts then need no second pass: with eps_x = D_a(i, j*) for
 // the selected frame j* of largest D_a, n_x + 1 = #{j : D_a(i, j) <= eps_x}
