@@ -1048,21 +1048,21 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 NSX_HIP(hipGetLastError(), "walk launch");
                 if (e1) (void)hipEventRecord(e1, stream);
                 // Every deferred query (no LDS: 8 workgroups of 4 waves per
-                // CU), then the exact ties among them with up to 4 waves per
-                // workgroup, each with its LDS row and lists.
+                // CU), then the exact ties among them: as many waves per CU
+                // as their LDS rows and lists allow (up to 8, one workgroup
+                // per CU; the waves take the entries in a grid stride).
                 hipLaunchKernelGGL(ws.ties, dim3((unsigned)tie_waves), dim3(ksgw::kWave * 4), 0,
                                    stream, wa);
                 NSX_HIP(hipGetLastError(), "tie select launch");
                 const size_t tie_wave_bytes = ksgw::tie_wave_lds_bytes(n);
-                const int tie_wpb =
-                    (int)std::max<size_t>(1, std::min<size_t>(4, lds_cap / tie_wave_bytes));
-                hipLaunchKernelGGL(ws.tiek, dim3((unsigned)(tie_waves / tie_wpb)),
-                                   dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
-                                   stream, wa);
+                const int tie_wpb = (int)std::max<size_t>(
+                    1, std::min<size_t>(ksgw::kTieMaxWaves, lds_cap / tie_wave_bytes));
+                const unsigned tie_groups = (unsigned)cu_count();
+                hipLaunchKernelGGL(ws.tiek, dim3(tie_groups), dim3(ksgw::kWave * tie_wpb),
+                                   (size_t)tie_wpb * tie_wave_bytes, stream, wa);
                 NSX_HIP(hipGetLastError(), "tie launch");
-                hipLaunchKernelGGL(ws.tieo, dim3((unsigned)(tie_waves / tie_wpb)),
-                                   dim3(ksgw::kWave * tie_wpb), (size_t)tie_wpb * tie_wave_bytes,
-                                   stream, wa);
+                hipLaunchKernelGGL(ws.tieo, dim3(tie_groups), dim3(ksgw::kWave * tie_wpb),
+                                   (size_t)tie_wpb * tie_wave_bytes, stream, wa);
                 NSX_HIP(hipGetLastError(), "tie overflow launch");
                 if (e2) (void)hipEventRecord(e2, stream);
                 }

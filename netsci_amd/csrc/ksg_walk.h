@@ -68,6 +68,7 @@ constexpr int kWave = 64;
 constexpr int kMaxFrames = 8192;  // 32 sort keys per lane, up to 4 waves per row
 constexpr int kBuildWaves = 4;
 constexpr int kMaxWalkWaves = 16;
+constexpr int kTieMaxWaves = 8;  // waves per workgroup of the exact-tie kernels
 
 // Waves per walk workgroup a k allows: with k+1 packed 32-bit heap
 // entries the walk fits 128 VGPRs (16 waves, 4 per SIMD) for every k up to
@@ -1631,7 +1632,7 @@ __global__ __launch_bounds__(256) void tie_select_kernel(WalkArgs a) {
 // with tie_wave_lds_bytes(n) of LDS per wave (the row of n records and the
 // partition's lists).
 template <int D, int K>
-__global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
+__global__ __launch_bounds__(kWave * kTieMaxWaves) void tie_kernel(WalkArgs a) {
     extern __shared__ __attribute__((aligned(16))) char tie_lds[];
     if (a.abort && *a.abort) return;
     const int lane = threadIdx.x & (kWave - 1);
@@ -1649,7 +1650,7 @@ __global__ __launch_bounds__(256) void tie_kernel(WalkArgs a) {
 // The tied queries the tie list had no room for (kTieMark in cnt): nothing
 // to do unless the list overflowed. Each wave scans 64-entry spans of cnt.
 template <int D, int K>
-__global__ __launch_bounds__(256) void tie_overflow_kernel(WalkArgs a) {
+__global__ __launch_bounds__(kWave * kTieMaxWaves) void tie_overflow_kernel(WalkArgs a) {
     if ((a.abort && *a.abort) || *a.tie_count <= a.tie_cap) return;
     extern __shared__ __attribute__((aligned(16))) char tie_lds[];
     const int lane = threadIdx.x & (kWave - 1);
