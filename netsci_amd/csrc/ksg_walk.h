@@ -1410,6 +1410,24 @@ __device__ int sort_prefix_wave(KeyIdx *a, int n, int m, uint16_t *lasc, uint16_
     return bound;
 }
 
+// Coordinates of frames j0 + u * 64 (u < U) of both nodes, every load issued
+// before any is used, so a row pass waits once per U frames per lane (the
+// tie kernels run few waves per CU). Frames past n read frame j0 (< n).
+template <int D, int U>
+__device__ __forceinline__ void load_frames(const float *xa, const float *xb, long long ds,
+                                            int j0, int n, float (&va)[U][D], float (&vb)[U][D]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * kWave;
+        const int jc = j < n ? j : j0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            va[u][d] = xa[d * ds + jc];
+            vb[u][d] = xb[d * ds + jc];
+        }
+    }
+}
+
 // Counts of one deferred query (local pair pl, frame i; equal k-th and
 // (k+1)-th heap codes), by one wave with an n-record LDS row: the wave
 // rebuilds the query's joint distance row. When the k-th and (k+1)-th
@@ -1440,21 +1458,23 @@ __device__ void resolve_tie(const WalkArgs &a, long long pl, int i, char *wave_l
     float h[K + 1];
 #pragma unroll
     for (int m = 0; m <= K; ++m) h[m] = __builtin_inff();
-    for (int j = lane; j < n; j += kWave) {
-        float va[D], vb[D];
+    for (int j0 = lane; j0 < n; j0 += 4 * kWave) {
+        float va[4][D], vb[4][D];
+        load_frames<D, 4>(xa, xb, a.dstride, j0, n, va, vb);
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            va[d] = xa[d * a.dstride + j];
-            vb[d] = xb[d * a.dstride + j];
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * kWave;
+            if (j < n) {
+                KeyIdx r;
+                r.key = fmaxf(marg<D>(qa, va[u]), marg<D>(qb, vb[u]));
+                r.idx = j;
+                row[j] = r;
+                const float c = j == i ? __builtin_inff() : r.key;
+#pragma unroll
+                for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+                h[0] = fminf(h[0], c);
+            }
         }
-        KeyIdx r;
-        r.key = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
-        r.idx = j;
-        row[j] = r;
-        const float c = j == i ? __builtin_inff() : r.key;
-#pragma unroll
-        for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
-        h[0] = fminf(h[0], c);
     }
     // Merge across the wave: afterwards every lane holds the row's k+1
     // smallest.
@@ -1548,18 +1568,18 @@ __device__ bool select_nearest(const WalkArgs &a, long long pl, int i, int lane)
     float h[K + 1];
 #pragma unroll
     for (int m = 0; m <= K; ++m) h[m] = __builtin_inff();
-    for (int j = lane; j < n; j += kWave) {
-        float va[D], vb[D];
+    for (int j0 = lane; j0 < n; j0 += 4 * kWave) {
+        float va[4][D], vb[4][D];
+        load_frames<D, 4>(xa, xb, a.dstride, j0, n, va, vb);
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            va[d] = xa[d * a.dstride + j];
-            vb[d] = xb[d * a.dstride + j];
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * kWave;
+            const float z = fmaxf(marg<D>(qa, va[u]), marg<D>(qb, vb[u]));
+            const float c = (j == i || j >= n) ? __builtin_inff() : z;
+#pragma unroll
+            for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
+            h[0] = fminf(h[0], c);
         }
-        const float z = fmaxf(marg<D>(qa, va), marg<D>(qb, vb));
-        const float c = j == i ? __builtin_inff() : z;
-#pragma unroll
-        for (int m = K; m >= 1; --m) h[m] = __builtin_amdgcn_fmed3f(h[m - 1], c, h[m]);
-        h[0] = fminf(h[0], c);
     }
 #pragma unroll
     for (int off = 1; off < kWave; off <<= 1) {
@@ -1576,19 +1596,19 @@ __device__ bool select_nearest(const WalkArgs &a, long long pl, int i, int lane)
     if (!(h[K - 1] < h[K])) return false;
     float ex = -1.0f, ey = -1.0f;
     int jx = 0, jy = 0;
-    for (int j = lane; j < n; j += kWave) {
-        float va[D], vb[D];
+    for (int j0 = lane; j0 < n; j0 += 4 * kWave) {
+        float va[4][D], vb[4][D];
+        load_frames<D, 4>(xa, xb, a.dstride, j0, n, va, vb);
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-            va[d] = xa[d * a.dstride + j];
-            vb[d] = xb[d * a.dstride + j];
+        for (int u = 0; u < 4; ++u) {
+            const int j = j0 + u * kWave;
+            const float da = marg<D>(qa, va[u]), db = marg<D>(qb, vb[u]);
+            const bool sel = j < n && j != i && fmaxf(da, db) <= h[K - 1];
+            jx = sel && da > ex ? j : jx;
+            ex = sel ? fmaxf(ex, da) : ex;
+            jy = sel && db > ey ? j : jy;
+            ey = sel ? fmaxf(ey, db) : ey;
         }
-        const float da = marg<D>(qa, va), db = marg<D>(qb, vb);
-        if (j == i || !(fmaxf(da, db) <= h[K - 1])) continue;
-        jx = da > ex ? j : jx;
-        ex = fmaxf(ex, da);
-        jy = db > ey ? j : jy;
-        ey = fmaxf(ey, db);
     }
     // Any frame of the largest marginal distance gives the same rank.
 #pragma unroll
