@@ -43,9 +43,10 @@
 //   * dense graphs: one WAVE per (i, j) tests 64 q at once against a
 //     transposed copy of A (coalesced) and a ballot picks the first hit.
 // A device flag written by the scan selects the kernel: the backtrack
-// launches both variants and the other one exits at once; the steps read
-// the flag of A's lists back once per call and launch only the one that
-// works (L launches that exit at once would cost more than the readback).
+// launches both variants and the other one exits at once; for the steps the
+// host polls the flag of A's lists (written to pinned host memory) once per
+// call and launches only the one that works (L launches that exit at once
+// would cost more than the wait).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,15 +61,6 @@ namespace {
 constexpr int kTile = 64;   // output tile edge
 constexpr int kKStep = 16;  // k values per LDS stage
 constexpr int kMicro = 4;   // outputs per thread per dimension
-
-__global__ void negzero_kernel(const float *__restrict__ A, long long count,
-                               unsigned *__restrict__ flag) {
-    long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    bool found = false;
-    for (; e < count; e += stride) found |= (__float_as_uint(A[e]) == 0x80000000u);
-    if (__ballot(found) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
-}
 
 // out[i][j] = min_k fl(A[j][k] + Hp[k][i]).
 template <bool kExactTies>
@@ -260,33 +252,51 @@ constexpr long long kDenseDegree = 128;
 
 __device__ __forceinline__ bool is_candidate(float a) { return a < __builtin_inff(); }
 
-// Candidate counts: one wave per node k scans row k of A^T (= column k of A)
-// and stores the list length rounded up to a multiple of 4.
-__global__ __launch_bounds__(256) void cand_count_kernel(const float *__restrict__ AT,
-                                                         long long *__restrict__ len4, int N) {
+// Candidate counts: one wave per row k of M stores the list length rounded
+// up to a multiple of 4, and in negz[k] whether the row holds a -0.0 (the
+// dense step's choice of fold; the scan ORs the rows).
+__global__ __launch_bounds__(256) void cand_count_kernel(const float *__restrict__ M,
+                                                         long long *__restrict__ len4,
+                                                         unsigned *__restrict__ negz, int N) {
     const int lane = threadIdx.x & 63;
     const int k = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (k >= N) return;
-    const float *row = AT + (long long)k * N;
+    const float *row = M + (long long)k * N;
     long long cnt = 0;
+    bool nz = false;
     for (int q0 = 0; q0 < N; q0 += 64) {
         const int q = q0 + lane;
-        const bool ok = q < N && is_candidate(row[q]);
-        cnt += __popcll(__ballot(ok));
+        const float a = q < N ? row[q] : 0.0f;
+        cnt += __popcll(__ballot(q < N && is_candidate(a)));
+        nz |= __float_as_uint(a) == 0x80000000u;
     }
-    if (lane == 0) len4[k] = (cnt + 3) & ~3LL;
+    const bool any_nz = __ballot(nz) != 0ull;
+    if (lane == 0) {
+        len4[k] = (cnt + 3) & ~3LL;
+        negz[k] = any_nz ? 1u : 0u;
+    }
 }
 
 // Exclusive scan of the padded lengths into rows[0..N] (one workgroup of
 // 1024 threads, 1024 nodes per round) and the dense/sparse flag.
+// dense[1]: the zero-diagonal flag that cand_fill_kernel clears (see there);
+// dense[2]: whether M holds a -0.0. With `hostf` (coherent pinned host
+// memory) the flags also go to hostf[1..2] and then `seq` to hostf[0], so
+// the host can poll for them instead of synchronising the stream.
 __global__ __launch_bounds__(1024) void cand_scan_kernel(const long long *__restrict__ len4,
+                                                         const unsigned *__restrict__ negz,
                                                          long long *__restrict__ rows,
                                                          unsigned *__restrict__ dense, int N,
-                                                         int allow_sparse) {
+                                                         int allow_sparse, unsigned *hostf,
+                                                         unsigned seq) {
     __shared__ long long s[1024];
+    __shared__ unsigned any_nz;
     const int t = threadIdx.x;
+    if (t == 0) any_nz = 0u;
+    __syncthreads();
     long long carry = 0;
     for (int base = 0; base < N; base += 1024) {
+        if (base + t < N && negz[base + t]) any_nz = 1u;
         s[t] = (base + t < N) ? len4[base + t] : 0;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -301,7 +311,16 @@ __global__ __launch_bounds__(1024) void cand_scan_kernel(const long long *__rest
     }
     if (t == 0) {
         rows[0] = 0;
-        dense[0] = (!allow_sparse || carry > kDenseDegree * (long long)N) ? 1u : 0u;
+        const unsigned d = (!allow_sparse || carry > kDenseDegree * (long long)N) ? 1u : 0u;
+        dense[0] = d;
+        dense[1] = 1u;
+        dense[2] = any_nz;
+        if (hostf) {
+            hostf[1] = d;
+            hostf[2] = any_nz;
+            __threadfence_system();
+            __hip_atomic_store(hostf, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
 }
 
@@ -317,10 +336,15 @@ __device__ __forceinline__ void put_entry(int *__restrict__ lst, long long e, in
     lst[g + 4 + slot] = __float_as_int(a);
 }
 
+// With `self` set (the backtrack's lists, M = A^T), row k also records the
+// index of its own entry q = k in self[k] (-1 when M[k][k] is not a
+// candidate) and clears dense[1] unless M[k][k] is +0.0: the backtrack's
+// gate needs every diagonal weight to be exactly +0.0.
 __global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict__ M,
                                                         const long long *__restrict__ rows,
-                                                        const unsigned *__restrict__ dense,
-                                                        int *__restrict__ lst, int N) {
+                                                        unsigned *__restrict__ dense,
+                                                        int *__restrict__ lst,
+                                                        int *__restrict__ self, int N) {
     if (dense[0]) return;
     const int lane = threadIdx.x & 63;
     const int k = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -328,6 +352,7 @@ __global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict_
     const float *row = M + (long long)k * N;
     const long long end = rows[k + 1];
     long long at = rows[k];
+    long long own = -1;  // index of the entry q = k (wave-uniform)
     for (int q0 = 0; q0 < N; q0 += 64) {
         const int q = q0 + lane;
         const float a = q < N ? row[q] : __builtin_inff();
@@ -336,7 +361,13 @@ __global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict_
         const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         if (ok) put_entry(lst, at + rank, q, a);
+        if (k >= q0 && k < q0 + 64 && ((m >> (k - q0)) & 1ull))
+            own = at + __popcll(m & ((1ull << (k - q0)) - 1ull));
         at += __popcll(m);
+    }
+    if (self && lane == 0) {
+        self[k] = (int)own;
+        if (__float_as_uint(row[k]) != 0u) atomicAnd(dense + 1, 0u);
     }
     for (long long e = at + lane; e < end; e += 64) put_entry(lst, e, 0, __builtin_inff());
 }
@@ -372,19 +403,20 @@ constexpr int kRowPerLane = kLdsRowMax / kBtThreads;
 typedef int nsx_v4i __attribute__((ext_vector_type(4)));
 
 // First entry of one list group {q0..q3 | a0..a3} that passes the test, in
-// slot order; -1 if none (or if `valid` is false). Branch-free: the four
+// slot order; -1 if none (or if `valid` is false). Slots below `smin` are
+// skipped (a scan that resumes inside the group). Branch-free: the four
 // tests are evaluated and a select chain keeps the lowest passing slot.
 template <bool kLdsRow>
 __device__ __forceinline__ int first_pass(const int4 q, const float4 a, const float *cur,
                                           const float *hrow, int j, float len, float tol,
-                                          bool valid, float &af) {
+                                          bool valid, int smin, float &af) {
     const float h0 = kLdsRow ? cur[q.x] : hrow[q.x];
     const float h1 = kLdsRow ? cur[q.y] : hrow[q.y];
     const float h2 = kLdsRow ? cur[q.z] : hrow[q.z];
     const float h3 = kLdsRow ? cur[q.w] : hrow[q.w];
-    const bool p0 = valid & (q.x != j) & ((h0 + a.x) - len < tol);
-    const bool p1 = valid & (q.y != j) & ((h1 + a.y) - len < tol);
-    const bool p2 = valid & (q.z != j) & ((h2 + a.z) - len < tol);
+    const bool p0 = valid & (smin <= 0) & (q.x != j) & ((h0 + a.x) - len < tol);
+    const bool p1 = valid & (smin <= 1) & (q.y != j) & ((h1 + a.y) - len < tol);
+    const bool p2 = valid & (smin <= 2) & (q.z != j) & ((h2 + a.z) - len < tol);
     const bool p3 = valid & (q.w != j) & ((h3 + a.w) - len < tol);
     int f = p3 ? q.w : -1;
     float fa = a.w;
@@ -407,13 +439,31 @@ __device__ __forceinline__ int first_pass(const int4 q, const float4 a, const fl
 // (needs a 16-byte aligned paths); otherwise one dword store per step.
 // kGroups: list groups whose loads are issued together before their tests
 // (loads past the list's end are skipped); the tests keep the slot order.
-template <bool kLdsRow, bool kLdsSpan, bool kQuad, int kGroups>
-__global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
+// kGate (with kLdsRow; active when every A[k][k] is +0.0, dense[1]): the
+// stay gate. With a zero diagonal, k = j is a candidate of every Jacobi
+// fold with weight 0, so H_{p+1}[i][j] <= H_p[j][i] and H_{p+2} <= H_p entry
+// by entry (entries of H_0 = A that are NaN pass no test and need no
+// bound): LB[q] = min(Hi[L-1][i][q], Hi[L-2][i][q]) is below Hi[p][i][q] for
+// every p. Rounding is monotone, so with
+//     bound[k] = min over q < k in cand(k) of fl(LB[q] + A[q][k])
+// (-inf if any such sum is NaN), fl(bound[k] - len) >= tol means that no
+// q < k passes the test at any step. The lane then tests q = k alone
+// (weight +0.0: len is unchanged when it passes) and, if that fails (or
+// k = j, which is never taken), resumes the scan after k's own entry
+// self[k]; otherwise it scans cand(k) from the start. Most steps of a path
+// that has reached its last hop count are such stays, so the gate replaces
+// their scan with two LDS reads. bound[] is computed per workgroup (one i)
+// into LDS after the row buffers and span table.
+template <bool kLdsRow, bool kLdsSpan, bool kQuad, int kGroups, bool kGate>
+__global__ __launch_bounds__(kBtThreads) __attribute__((amdgpu_waves_per_eu(8))) void
+backtrack_sparse_kernel(
     const float *__restrict__ H, const float *__restrict__ Hi, const long long *__restrict__ rows,
-    const int *__restrict__ lst, const unsigned *__restrict__ dense, int *__restrict__ paths,
-    float tol, int N, int L, int nC) {
-    extern __shared__ float hrow_lds[];  // kLdsRow: 2 x N floats; kLdsSpan: then N + 1 ints
+    const int *__restrict__ lst, const unsigned *__restrict__ dense,
+    const int *__restrict__ self, int *__restrict__ paths, float tol, int N, int L, int nC) {
+    // kLdsRow: 2 x N floats; kLdsSpan: then N + 1 ints; kGate: then N floats
+    extern __shared__ float hrow_lds[];
     if (dense[0]) return;
+    const bool gate = kGate && dense[1] != 0u;
     const int g = blockIdx.x;
     const int s = g >> 3;
     const int i = (g & 7) + 8 * (s / nC);
@@ -428,25 +478,74 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
         for (int p = 0; p < L; ++p) pp[p] = i;
     float len = active ? H[(long long)i * N + j] : 0.0f;
     int k = j;
+    // Row staging: 16-byte loads and LDS writes when rows are 16-byte aligned.
+    const bool v4 = (N & 3) == 0 && ((uintptr_t)Hi & 15) == 0;
     float next[kRowPerLane];
-    int *span_lds = reinterpret_cast<int *>(hrow_lds + 2 * N);
+    int *span_lds = reinterpret_cast<int *>(hrow_lds + (kLdsRow ? 2 * N : 0));
+    float *bound_lds = hrow_lds + (kLdsRow ? 2 * N : 0) + (kLdsSpan ? N + 1 : 0);
     if (kLdsSpan)
         for (int t = t0; t <= N; t += kBtThreads) span_lds[t] = (int)(rows[t] >> 2);
     if (kLdsRow) {
         const float *row = Hi + (long long)(L - 1) * NN + (long long)i * N;
         for (int t = t0; t < N; t += kBtThreads) hrow_lds[t] = row[t];
+        // The gate's bound reads Hi[L-2][i][.] too; step L-1 stages the same
+        // row into this buffer again.
+        if (gate && L >= 2)
+            for (int t = t0; t < N; t += kBtThreads) hrow_lds[N + t] = row[t - NN];
     }
     if (kLdsRow || kLdsSpan) __syncthreads();
+    if (gate) {
+        const float inf = __builtin_inff();
+        const float *last = Hi + (long long)(L - 1) * NN + (long long)i * N;  // Hi[L-1][i][.]
+        const float *r0 = kLdsRow ? hrow_lds : last;
+        const float *r1 = kLdsRow ? hrow_lds + N : (L >= 2 ? last - NN : last);
+        for (int kk = t0; kk < N; kk += kBtThreads) {
+            float m = inf;
+            const int own = self[kk];  // >= 0: the zero diagonal makes kk a candidate
+            int e = kLdsSpan ? 4 * span_lds[kk] : (int)rows[kk];
+            for (; e < own; e += 4) {
+                const int *grp = lst + 2 * (size_t)(unsigned)e;  // group e / 4 (e % 4 == 0)
+                const int4 q = *reinterpret_cast<const int4 *>(grp);
+                const float4 a = *reinterpret_cast<const float4 *>(grp + 4);
+                const int qs[4] = {q.x, q.y, q.z, q.w};
+                const float as[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (e + u >= own) break;
+                    const float lb =
+                        L >= 2 ? __builtin_fminf(r0[qs[u]], r1[qs[u]]) : r0[qs[u]];
+                    const float t = lb + as[u];
+                    m = (t != t) ? -inf : __builtin_fminf(m, t);
+                }
+            }
+            bound_lds[kk] = m;
+        }
+        __syncthreads();
+    }
     nsx_v4i quad = {0, 0, 0, 0};
     for (int p = L - 1; p >= 0; --p) {
         const float *hrow = Hi + (long long)p * NN + (long long)i * N;
         const float *cur = hrow_lds + ((L - 1 - p) & 1) * N;
         if (kLdsRow && p > 0) {
             const float *row = hrow - NN;  // Hi[p-1][i][.]
+            if (v4) {
 #pragma unroll
-            for (int r = 0; r < kRowPerLane; ++r) {
-                const int t = t0 + r * kBtThreads;
-                next[r] = t < N ? row[t] : 0.0f;
+                for (int r = 0; r < kRowPerLane / 4; ++r) {
+                    const int t = 4 * (t0 + r * kBtThreads);
+                    if (t < N) {
+                        const float4 v = *reinterpret_cast<const float4 *>(row + t);
+                        next[4 * r] = v.x;
+                        next[4 * r + 1] = v.y;
+                        next[4 * r + 2] = v.z;
+                        next[4 * r + 3] = v.w;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < kRowPerLane; ++r) {
+                    const int t = t0 + r * kBtThreads;
+                    next[r] = t < N ? row[t] : 0.0f;
+                }
             }
         }
         if (active) {
@@ -461,7 +560,18 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
             }
             int found = -1;
             float af = 0.0f;
-            for (int gi = gb; gi < ge && found < 0; gi += kGroups) {
+            int est = 4 * gb;  // first entry the scan tests
+            bool scan = true;
+            if (gate && bound_lds[k] - len >= tol) {
+                const float hk = kLdsRow ? cur[k] : hrow[k];
+                if (k != j && (hk + 0.0f) - len < tol) {
+                    found = k;  // stay: weight A[k][k] = +0.0
+                    scan = false;
+                } else {
+                    est = self[k] + 1;
+                }
+            }
+            for (int gi = est >> 2; scan && gi < ge && found < 0; gi += kGroups) {
                 // Groups past the list's end are read at the last group of
                 // the list (in bounds) and masked out.
                 int4 q[kGroups];
@@ -477,7 +587,7 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
                 for (int u = 0; u < kGroups; ++u) {
                     float au;
                     const int fu = first_pass<kLdsRow>(q[u], a[u], cur, hrow, j, len, tol,
-                                                       gi + u < ge, au);
+                                                       gi + u < ge, est - 4 * (gi + u), au);
                     const bool take = found < 0 && fu >= 0;
                     af = take ? au : af;
                     found = take ? fu : found;
@@ -510,10 +620,21 @@ __global__ __launch_bounds__(kBtThreads) void backtrack_sparse_kernel(
         }
         if (kLdsRow && p > 0) {
             float *nxt = hrow_lds + ((L - p) & 1) * N;
+            if (v4) {
 #pragma unroll
-            for (int r = 0; r < kRowPerLane; ++r) {
-                const int t = t0 + r * kBtThreads;
-                if (t < N) nxt[t] = next[r];
+                for (int r = 0; r < kRowPerLane / 4; ++r) {
+                    const int t = 4 * (t0 + r * kBtThreads);
+                    if (t < N)
+                        *reinterpret_cast<float4 *>(nxt + t) =
+                            make_float4(next[4 * r], next[4 * r + 1], next[4 * r + 2],
+                                        next[4 * r + 3]);
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < kRowPerLane; ++r) {
+                    const int t = t0 + r * kBtThreads;
+                    if (t < N) nxt[t] = next[r];
+                }
             }
             __syncthreads();  // scans of `cur` done, `nxt` complete
         }
@@ -584,34 +705,41 @@ __global__ void corr_to_adj_kernel(const float *__restrict__ C, float *__restric
 struct Lists {
     long long *rows = nullptr;
     int *lst = nullptr;  // groups of 4 entries: 4 node ids, then 4 weights
-    unsigned *dense = nullptr;
-    bool host_dense = false;  // *dense, when read back (step lists only)
+    unsigned *dense = nullptr;  // [0] dense flag, [1] zero-diagonal flag (backtrack lists)
+    int *self = nullptr;        // backtrack lists: index of node k's own entry
+    bool host_dense = false;  // dense[0], when read back (step lists only)
 };
 
 int build_lists(const float *M, int N, int slot_rows, int slot_cand, bool allow_sparse,
-                hipStream_t s, Lists &l) {
-    l.rows = (long long *)workspace(slot_rows, (size_t)(2 * N + 2) * sizeof(long long));
+                bool want_self, unsigned *hostf, unsigned seq, hipStream_t s, Lists &l) {
+    // rows[0..N], len4[0..N), 4 flag words, self[0..N), negz[0..N)
+    l.rows = (long long *)workspace(
+        slot_rows, (size_t)(2 * N + 1) * sizeof(long long) + 16 + (size_t)N * 2 * sizeof(int));
     if (!l.rows) return kHipError;
     long long *len4 = l.rows + N + 1;
     l.dense = (unsigned *)(len4 + N);
+    int *self = (int *)(l.dense + 4);
+    unsigned *negz = (unsigned *)(self + N);
+    l.self = want_self ? self : nullptr;
     const size_t cap = ((size_t)N * (size_t)(N + 3) + 3) & ~(size_t)3;
     l.lst = (int *)workspace(slot_cand, cap * (sizeof(int) + sizeof(float)));
     if (!l.lst) return kHipError;
     const int wb = (N + 3) / 4;
-    hipLaunchKernelGGL(cand_count_kernel, dim3(wb), dim3(256), 0, s, M, len4, N);
+    hipLaunchKernelGGL(cand_count_kernel, dim3(wb), dim3(256), 0, s, M, len4, negz, N);
     NSX_HIP(hipGetLastError(), "candidate count launch");
-    hipLaunchKernelGGL(cand_scan_kernel, dim3(1), dim3(1024), 0, s, len4, l.rows, l.dense, N,
-                       allow_sparse ? 1 : 0);
+    hipLaunchKernelGGL(cand_scan_kernel, dim3(1), dim3(1024), 0, s, len4, negz, l.rows,
+                       l.dense, N, allow_sparse ? 1 : 0, hostf, seq);
     NSX_HIP(hipGetLastError(), "candidate scan launch");
-    hipLaunchKernelGGL(cand_fill_kernel, dim3(wb), dim3(256), 0, s, M, l.rows, l.dense, l.lst, N);
+    hipLaunchKernelGGL(cand_fill_kernel, dim3(wb), dim3(256), 0, s, M, l.rows, l.dense, l.lst,
+                       l.self, N);
     NSX_HIP(hipGetLastError(), "candidate fill launch");
     return kOk;
 }
 
 // One Jacobi step out = step(Hp): the sparse or the dense kernel, as the
 // dense flag of A's neighbour lists (read back once per call) says.
-int minplus_step(const float *A, const float *Hp, float *out, int N, const unsigned *negzero,
-                 const Lists &la, hipStream_t s) {
+int minplus_step(const float *A, const float *Hp, float *out, int N, const Lists &la,
+                 hipStream_t s) {
     if (!la.host_dense) {
         const int nI = (N + 63) / 64;
         // 64-wide j tiles when they give every CU at least four workgroups.
@@ -630,41 +758,47 @@ int minplus_step(const float *A, const float *Hp, float *out, int N, const unsig
     }
     const int tiles = (N + kTile - 1) / kTile;
     hipLaunchKernelGGL(minplus_kernel, dim3(tiles, tiles), dim3(256), 0, s, A, Hp, out, N,
-                       negzero, (const unsigned *)la.dense);
+                       la.dense + 2, (const unsigned *)la.dense);
     NSX_HIP(hipGetLastError(), "min-plus launch");
     return kOk;
 }
 
 
-int negzero_flag(const float *A, int N, unsigned **flag, hipStream_t s) {
-    unsigned *f = (unsigned *)workspace(kWsApsp1, 256);
-    if (!f) return kHipError;
-    int rc = fill_u32(f, 0u, 1, s);
+// Shared set-up of both entry points: A's neighbour lists with the dense
+// flag (dense[0]) and the negative-zero flag (dense[2]). The host needs
+// dense[0] to launch only the step kernel that does the work (a launch that
+// exits at once costs a few microseconds of a step of about ten at
+// N = 1000): the scan kernel writes the flags to coherent pinned host memory
+// and the host polls them, which returns tens of microseconds sooner than a
+// stream synchronisation; if the stream completes without them (or the
+// poll budget runs out) they are read back with a copy.
+int step_setup(const float *A, int N, hipStream_t s, Lists &la) {
+    static unsigned seq = 0;
+    unsigned *hostf = nullptr, *devf = nullptr;
+    int rc = host_flags(&hostf, &devf);
     if (rc) return rc;
-    const long long count = (long long)N * N;
-    const int blocks = (int)std::min<long long>((count + 255) / 256, 2048);
-    hipLaunchKernelGGL(negzero_kernel, dim3(blocks), dim3(256), 0, s, A, count, f);
-    NSX_HIP(hipGetLastError(), "negative-zero scan launch");
-    *flag = f;
-    return kOk;
-}
-
-// Shared set-up of both entry points: the negative-zero flag (dense step)
-// and A's neighbour lists with the dense/sparse flag.
-int step_setup(const float *A, int N, hipStream_t s, unsigned **negzero, Lists &la) {
-    int rc = negzero_flag(A, N, negzero, s);
+    const unsigned my = ++seq;
+    rc = build_lists(A, N, kWsApspRowsA, kWsApspCandA, true, false, devf, my, s, la);
     if (rc) return rc;
-    rc = build_lists(A, N, kWsApspRowsA, kWsApspCandA, true, s, la);
-    if (rc) return rc;
-    // One readback per call: each of the L steps then launches only the
-    // kernel that does the work (a launch that exits at once still costs
-    // a few microseconds of a step of about ten at N = 1000).
-    unsigned h_dense = 0;
-    NSX_HIP(hipMemcpyAsync(&h_dense, la.dense, sizeof(unsigned), hipMemcpyDeviceToHost, s),
-            "dense flag readback");
-    NSX_HIP(hipStreamSynchronize(s), "dense flag sync");
-    la.host_dense = h_dense != 0;
-    return kOk;
+    volatile unsigned *hf = hostf;
+    for (long long it = 0;; ++it) {
+        if (__atomic_load_n(hostf, __ATOMIC_ACQUIRE) == my) {
+            la.host_dense = hf[1] != 0;
+            return kOk;
+        }
+        if ((it & 255) == 255) {
+            const hipError_t e = hipStreamQuery(s);
+            if (e == hipErrorNotReady && it < (1LL << 26)) continue;
+            if (e != hipSuccess && e != hipErrorNotReady) NSX_HIP(e, "list set-up");
+            if (__atomic_load_n(hostf, __ATOMIC_ACQUIRE) == my) continue;
+            unsigned h_flags[4] = {0, 0, 0, 0};
+            NSX_HIP(hipMemcpyAsync(h_flags, la.dense, sizeof(h_flags), hipMemcpyDeviceToHost, s),
+                    "dense flag readback");
+            NSX_HIP(hipStreamSynchronize(s), "dense flag sync");
+            la.host_dense = h_flags[0] != 0;
+            return kOk;
+        }
+    }
 }
 
 }  // namespace
@@ -682,20 +816,27 @@ int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *str
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     const size_t bytes = (size_t)N * N * sizeof(float);
-    unsigned *negzero = nullptr;
     Lists la;
-    rc = step_setup(d_A, N, s, &negzero, la);
+    rc = step_setup(d_A, N, s, la);
     if (rc) return rc;
     float *tmp = (float *)workspace(kWsApsp0, bytes);
     if (!tmp) return kHipError;
-    // Ping-pong so that the last iteration lands in d_H.
-    float *bufs[2] = {d_H, tmp};
-    int cur = (L % 2 == 0) ? 0 : 1;
-    NSX_HIP(hipMemcpyAsync(bufs[cur], d_A, bytes, hipMemcpyDeviceToDevice, s), "H0 = A");
+    // Step p writes d_H when L-1-p is even and tmp otherwise, so the last
+    // step lands in d_H; step 0 reads A itself. If A and H overlap, A is
+    // copied into the buffer step 0 does not write first.
+    const bool overlap = d_A < d_H + (size_t)N * N && d_H < d_A + (size_t)N * N;
+    const float *in = d_A;
+    if (overlap || L == 0) {
+        float *h0 = (L % 2 == 0) ? d_H : tmp;
+        if (h0 != d_A)
+            NSX_HIP(hipMemcpyAsync(h0, d_A, bytes, hipMemcpyDeviceToDevice, s), "H0 = A");
+        in = h0;
+    }
     for (int p = 0; p < L; ++p) {
-        rc = minplus_step(d_A, bufs[cur], bufs[cur ^ 1], N, negzero, la, s);
+        float *out = ((L - 1 - p) % 2 == 0) ? d_H : tmp;
+        rc = minplus_step(d_A, in, out, N, la, s);
         if (rc) return rc;
-        cur ^= 1;
+        in = out;
     }
     return kOk;
 }
@@ -713,14 +854,13 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     hipStream_t s = (hipStream_t)stream;
     const long long NN = (long long)N * N;
     const size_t bytes = (size_t)NN * sizeof(float);
-    unsigned *negzero = nullptr;
     Lists la;
-    rc = step_setup(d_A, N, s, &negzero, la);
+    rc = step_setup(d_A, N, s, la);
     if (rc) return rc;
     NSX_HIP(hipMemcpyAsync(d_Hi, d_A, bytes, hipMemcpyDeviceToDevice, s), "Hi[0] = A");
     for (int p = 0; p < L; ++p) {
         float *out = (p + 1 < L) ? d_Hi + (size_t)(p + 1) * NN : d_H;
-        rc = minplus_step(d_A, d_Hi + (size_t)p * NN, out, N, negzero, la, s);
+        rc = minplus_step(d_A, d_Hi + (size_t)p * NN, out, N, la, s);
         if (rc) return rc;
     }
     if (!d_paths) return kOk;
@@ -733,21 +873,24 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
                        (const unsigned *)nullptr);
     NSX_HIP(hipGetLastError(), "transpose launch");
     Lists lt;
-    rc = build_lists(AT, N, kWsApspRows, kWsApspCand, true, s, lt);
+    rc = build_lists(AT, N, kWsApspRows, kWsApspCand, true, true, nullptr, 0u, s, lt);
     if (rc) return rc;
     const int nC = (N + kBtThreads - 1) / kBtThreads;
     const int bt_groups = 8 * ((N + 7) / 8) * nC;
     const size_t row_lds = 2 * (size_t)N * sizeof(float);
     const size_t span_lds = row_lds + (size_t)(N + 1) * sizeof(int);
+    const size_t bound_bytes = (size_t)N * sizeof(float);
     const bool quad = ((uintptr_t)d_paths & 15) == 0;
-#define NSX_BT(R, S, W, G, lds)                                                                  \
-    hipLaunchKernelGGL((backtrack_sparse_kernel<R, S, W, G>), dim3(bt_groups), dim3(kBtThreads), \
-                       lds, s, d_H, d_Hi, lt.rows, lt.lst, lt.dense, d_paths, tol, N, L, nC)
-    if (N > kLdsRowMax && quad) NSX_BT(false, false, true, 2, 0);
-    else if (N > kLdsRowMax) NSX_BT(false, false, false, 2, 0);
-    else if (!quad) NSX_BT(true, false, false, 2, row_lds);
-    else if (span_lds > 65536) NSX_BT(true, false, true, 2, row_lds);
-    else NSX_BT(true, true, true, 2, span_lds);
+#define NSX_BT(R, S, W, G, T, lds)                                                       \
+    hipLaunchKernelGGL((backtrack_sparse_kernel<R, S, W, G, T>), dim3(bt_groups),        \
+                       dim3(kBtThreads), lds, s, d_H, d_Hi, lt.rows, lt.lst, lt.dense, \
+                       lt.self, d_paths, tol, N, L, nC)
+    // LDS: up to 80 KB keeps two workgroups per CU.
+    if (N > kLdsRowMax && quad) NSX_BT(false, false, true, 2, false, 0);
+    else if (N > kLdsRowMax) NSX_BT(false, false, false, 2, false, 0);
+    else if (!quad) NSX_BT(true, false, false, 2, false, row_lds);
+    else if (span_lds + bound_bytes <= 80 * 1024) NSX_BT(true, true, true, 2, true, span_lds + bound_bytes);
+    else NSX_BT(true, false, true, 2, true, row_lds + bound_bytes);
 #undef NSX_BT
     NSX_HIP(hipGetLastError(), "sparse backtrack launch");
     hipLaunchKernelGGL(backtrack_kernel, dim3(N), dim3(256), 0, s, d_H, d_Hi, AT, lt.dense,

@@ -98,6 +98,11 @@ enum WorkspaceSlot : int {
     kWsCount = 20,
 };
 
+// 64 coherent pinned host words (allocated once per process): `host` for the
+// host, `dev` for kernels. Kernels write results there that the host polls
+// instead of waiting for a stream synchronisation.
+int host_flags(unsigned **host, unsigned **dev);
+
 // Fill `count` 32-bit words at d_ptr with `value` on `stream` (a kernel, so
 // it may be recorded into a HIP graph).
 int fill_u32(void *d_ptr, uint32_t value, size_t count, hipStream_t stream);

@@ -179,6 +179,28 @@ static int stage_init() {
     return kOk;
 }
 
+// Coherent pinned host words that kernels write and the host polls.
+static std::mutex g_flag_mutex;
+static unsigned *g_flag_host = nullptr;
+static unsigned *g_flag_dev = nullptr;
+
+int host_flags(unsigned **host, unsigned **dev) {
+    std::lock_guard<std::mutex> lock(g_flag_mutex);
+    if (!g_flag_host) {
+        void *p = nullptr;
+        NSX_HIP(hipHostMalloc(&p, 256, hipHostMallocMapped | hipHostMallocCoherent),
+                "flag hipHostMalloc");
+        void *d = nullptr;
+        NSX_HIP(hipHostGetDevicePointer(&d, p, 0), "flag device pointer");
+        std::memset(p, 0, 256);
+        g_flag_host = (unsigned *)p;
+        g_flag_dev = (unsigned *)d;
+    }
+    *host = g_flag_host;
+    *dev = g_flag_dev;
+    return kOk;
+}
+
 }  // namespace nsx
 
 using namespace nsx;

@@ -164,6 +164,32 @@ def test_sparse_steps_mixed_tiles():
     np.testing.assert_array_equal(dHi.numpy().view(np.uint32), Hio.view(np.uint32))
 
 
+@pytest.mark.parametrize("N,L,diag", [(160, 11, "zero"), (160, 11, "positive"),
+                                      (160, 11, "negzero"), (96, 1, "zero"), (96, 2, "zero"),
+                                      (700, 9, "zero")])
+def test_backtrack_stay_gate(N, L, diag):
+    """Directed graphs (H_p far from symmetric, so the gate's bound needs
+    both H_{L-1} and H_{L-2}) with a band of short edges, whose paths mostly
+    stay at a node for many steps. With an all-+0.0 diagonal the backtrack
+    skips the scans the gate rules out; a positive or -0.0 diagonal entry
+    turns the gate off. Paths must equal the oracle in every case."""
+    print(f"apsp stay gate N={N} L={L} diag={diag}", flush=True)
+    rng = np.random.default_rng(N + L)
+    A = rng.uniform(0.05, 2.0, size=(N, N)).astype(np.float32)
+    band = np.abs(np.arange(N)[:, None] - np.arange(N)[None, :]) <= 3
+    keep = (band & (rng.random((N, N)) < 0.9)) | (rng.random((N, N)) < 0.01)
+    A[~keep] = np.inf
+    np.fill_diagonal(A, 0.0)
+    if diag == "positive":
+        A[5, 5] = 0.25
+    elif diag == "negzero":
+        A[7, 7] = -0.0
+    H, Hi, P = _gpu(A, L, 1e-4)
+    Ho, Hio, Po = oracle.hedetniemi(A, L, 1e-4, want_hi=True)
+    np.testing.assert_array_equal(H.view(np.uint32), Ho.view(np.uint32))
+    np.testing.assert_array_equal(P, Po)
+
+
 def test_lengths_only_matches():
     N, L = 150, 9
     print("apsp lengths", flush=True)
@@ -172,6 +198,23 @@ def test_lengths_only_matches():
     dH = abi.DeviceArray((N, N), np.float32)
     abi.hedetniemi_lengths(dA, N, L, dH)
     Ho, _, _ = oracle.hedetniemi(A, L, 0.0, want_paths=False)
+    np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
+
+
+@pytest.mark.parametrize("L,in_place", [(0, False), (1, False), (4, False), (3, True),
+                                         (4, True)])
+def test_lengths_first_step_reads_A(L, in_place):
+    """The lengths entry point's first step reads A itself (no H_0 copy);
+    when H is A's own buffer, A is copied into the buffer that step 0 does
+    not write. Both parities of L, L = 0 (H = A) and L = 1, against the
+    oracle, with -0.0 weights so the exact fold runs."""
+    N = 140
+    print(f"apsp lengths L={L} in_place={in_place}", flush=True)
+    A = _graph(N, 0.05, seed=50 + L, negzero=True)
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = dA if in_place else abi.DeviceArray((N, N), np.float32)
+    abi.hedetniemi_lengths(dA, N, L, dH)
+    Ho, _, _ = oracle.hedetniemi(A, L, 0.0, want_paths=False) if L else (A, None, None)
     np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
 
 
