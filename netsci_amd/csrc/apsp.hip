@@ -381,11 +381,11 @@ struct __attribute__((aligned(8))) Span {
 // one i and 1024 consecutive j. Every step p all its lanes read from the row
 // Hi[p][i][.], so the workgroup stages that row in LDS (kLdsRow; N <=
 // kLdsRowMax) and the per-candidate reads become LDS reads. Two row buffers:
-// while the lanes scan step p from one, each lane's share of row p-1 is
-// already loading into registers and goes to the other buffer afterwards,
-// so one barrier per step separates the scans and the row fetch is hidden
-// behind them. Workgroups g, g+8, ... share an XCD and get the same i, so the
-// row comes from that XCD's L2 for all but the first of them. Writes
+// while the lanes scan step p from one, row p-1 streams into the other by
+// LDS DMA (16-byte rows; otherwise it is copied after the scan), so one
+// barrier per step separates the scans and the row fetch is hidden behind
+// them. Workgroups g, g+8, ... share an XCD and get the same i, so the row
+// comes from that XCD's L2 for all but the first of them. Writes
 // paths[i][j][0..L-1] (diagonal: i).
 // Measured at N = 4000, L = 132 (7.5 A contact graph): one dword store per
 // step 58.7 ms; non-temporal dword stores 109 ms; 16-byte non-temporal
@@ -394,7 +394,11 @@ struct __attribute__((aligned(8))) Span {
 // which halves the resident waves: 54.8 ms. Tests of a group evaluated
 // without branches (a select chain keeps the first passing slot; the
 // per-entry early exits compiled to nested exec-mask branches) 35.7 ms;
-// with that, three groups 36.6 ms and four 44.0 ms.
+// with that, three groups 36.6 ms and four 44.0 ms. Later, with the stay
+// gate (24.2 -> 22.6 ms): 32-byte blocks 15.7 ms (the 16-byte stores were
+// written to memory as 32-byte sectors: 16.5 GB for 8.45 GB of entries;
+// 10.2 GB with 32-byte blocks); 64-byte blocks and one list group in flight
+// 15.2 ms; the next row by LDS DMA instead of through registers 13.5 ms.
 constexpr int kBtThreads = 1024;
 constexpr int kMaxNodes = 92680;  // N * (N + 3) / 4 list groups fit an int
 constexpr int kLdsRowMax = 8192;
@@ -433,10 +437,11 @@ __device__ __forceinline__ int first_pass(const int4 q, const float4 a, const fl
 // kLdsRow: Hi[p][i][.] staged in LDS (two row buffers, N <= kLdsRowMax).
 // kLdsSpan: the list starts (in groups of 4 entries) of every node follow
 // the row buffers in LDS, so a step's list lookup is an LDS read.
-// kQuad: steps whose paths entries fill an aligned 16-byte block inside the
-// lane's own paths[i][j][0..L) are collected in registers and written as one
-// 16-byte non-temporal store, the partial blocks at either end as dwords
-// (needs a 16-byte aligned paths); otherwise one dword store per step.
+// kQuad: steps whose paths entries fill an aligned 64-byte block (16
+// entries) inside the lane's own paths[i][j][0..L) are collected in
+// registers and written as four 16-byte non-temporal stores, the partial
+// blocks at either end as dwords (needs a 16-byte aligned paths); otherwise
+// one dword store per step.
 // kGroups: list groups whose loads are issued together before their tests
 // (loads past the list's end are skipped); the tests keep the slot order.
 // kGate (with kLdsRow; active when every A[k][k] is +0.0, dense[1]): the
@@ -480,7 +485,6 @@ backtrack_sparse_kernel(
     int k = j;
     // Row staging: 16-byte loads and LDS writes when rows are 16-byte aligned.
     const bool v4 = (N & 3) == 0 && ((uintptr_t)Hi & 15) == 0;
-    float next[kRowPerLane];
     int *span_lds = reinterpret_cast<int *>(hrow_lds + (kLdsRow ? 2 * N : 0));
     float *bound_lds = hrow_lds + (kLdsRow ? 2 * N : 0) + (kLdsSpan ? N + 1 : 0);
     if (kLdsSpan)
@@ -522,30 +526,27 @@ backtrack_sparse_kernel(
         }
         __syncthreads();
     }
-    nsx_v4i quad = {0, 0, 0, 0};
+    nsx_v4i quad[4] = {};
     for (int p = L - 1; p >= 0; --p) {
         const float *hrow = Hi + (long long)p * NN + (long long)i * N;
         const float *cur = hrow_lds + ((L - 1 - p) & 1) * N;
-        if (kLdsRow && p > 0) {
-            const float *row = hrow - NN;  // Hi[p-1][i][.]
-            if (v4) {
+        if (kLdsRow && p > 0 && v4) {
+            // Hi[p-1][i][.] goes straight into the other row buffer while the
+            // lanes scan this step (global_load_lds_dwordx4: 16 bytes per lane
+            // land at the wave's LDS base + 16 x lane), so no registers are
+            // held across the scan. The wait before the step's barrier
+            // completes it.
+            const float *row = hrow - NN;
+            float *nxt = hrow_lds + ((L - p) & 1) * N;
+            const int wv = t0 >> 6;
 #pragma unroll
-                for (int r = 0; r < kRowPerLane / 4; ++r) {
-                    const int t = 4 * (t0 + r * kBtThreads);
-                    if (t < N) {
-                        const float4 v = *reinterpret_cast<const float4 *>(row + t);
-                        next[4 * r] = v.x;
-                        next[4 * r + 1] = v.y;
-                        next[4 * r + 2] = v.z;
-                        next[4 * r + 3] = v.w;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < kRowPerLane; ++r) {
-                    const int t = t0 + r * kBtThreads;
-                    next[r] = t < N ? row[t] : 0.0f;
-                }
+            for (int r = 0; r < kRowPerLane / 4; ++r) {
+                const int tb = 4 * (64 * wv + r * kBtThreads);  // the wave's first float
+                const int t = 4 * (t0 + r * kBtThreads);        // this lane's (t < N: t + 3 < N)
+                if (tb < N && t < N)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(row + t),
+                        (__attribute__((address_space(3))) void *)(nxt + tb), 16, 0, 0);
             }
         }
         if (active) {
@@ -599,18 +600,21 @@ backtrack_sparse_kernel(
             }
             if (kQuad) {
                 // a = index of paths[i][j][p] in the int array; its aligned
-                // block [a & ~3, (a & ~3) + 3] lies inside this lane's
-                // entries [base, base + L) or is written entry by entry.
+                // 16-entry block [a & ~15, (a & ~15) + 15] lies inside this
+                // lane's entries [base, base + L) or is written entry by entry.
                 const long long a = base + p;
-                const long long b0 = a & ~3LL;
-                const int slot = (int)(a & 3);
-                if (b0 >= base && b0 + 3 < base + L) {
-                    quad.x = slot == 0 ? k : quad.x;
-                    quad.y = slot == 1 ? k : quad.y;
-                    quad.z = slot == 2 ? k : quad.z;
-                    quad.w = slot == 3 ? k : quad.w;
-                    if (slot == 0)
-                        __builtin_nontemporal_store(quad, reinterpret_cast<nsx_v4i *>(paths + b0));
+                const long long b0 = a & ~15LL;
+                const int slot = (int)(a & 15);
+                if (b0 >= base && b0 + 15 < base + L) {
+                    int *blk = reinterpret_cast<int *>(&quad);
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) blk[u] = slot == u ? k : blk[u];
+                    if (slot == 0) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            __builtin_nontemporal_store(
+                                quad[u], reinterpret_cast<nsx_v4i *>(paths + b0 + 4 * u));
+                    }
                 } else {
                     pp[p] = k;
                 }
@@ -619,22 +623,12 @@ backtrack_sparse_kernel(
             }
         }
         if (kLdsRow && p > 0) {
-            float *nxt = hrow_lds + ((L - p) & 1) * N;
             if (v4) {
-#pragma unroll
-                for (int r = 0; r < kRowPerLane / 4; ++r) {
-                    const int t = 4 * (t0 + r * kBtThreads);
-                    if (t < N)
-                        *reinterpret_cast<float4 *>(nxt + t) =
-                            make_float4(next[4 * r], next[4 * r + 1], next[4 * r + 2],
-                                        next[4 * r + 3]);
-                }
+                __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this lane's row DMA landed
             } else {
-#pragma unroll
-                for (int r = 0; r < kRowPerLane; ++r) {
-                    const int t = t0 + r * kBtThreads;
-                    if (t < N) nxt[t] = next[r];
-                }
+                const float *row = hrow - NN;  // Hi[p-1][i][.]
+                float *nxt = hrow_lds + ((L - p) & 1) * N;
+                for (int t = t0; t < N; t += kBtThreads) nxt[t] = row[t];
             }
             __syncthreads();  // scans of `cur` done, `nxt` complete
         }
@@ -889,7 +883,8 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     if (N > kLdsRowMax && quad) NSX_BT(false, false, true, 2, false, 0);
     else if (N > kLdsRowMax) NSX_BT(false, false, false, 2, false, 0);
     else if (!quad) NSX_BT(true, false, false, 2, false, row_lds);
-    else if (span_lds + bound_bytes <= 80 * 1024) NSX_BT(true, true, true, 2, true, span_lds + bound_bytes);
+    else if (span_lds + bound_bytes <= 80 * 1024)
+        NSX_BT(true, true, true, 1, true, span_lds + bound_bytes);
     else NSX_BT(true, false, true, 2, true, row_lds + bound_bytes);
 #undef NSX_BT
     NSX_HIP(hipGetLastError(), "sparse backtrack launch");

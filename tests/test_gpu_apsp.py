@@ -95,7 +95,7 @@ def test_contact_graph_paths_bitexact():
 @pytest.mark.parametrize("L", [7, 8])
 def test_paths_unaligned_buffer_bitexact(L):
     """A paths pointer that is not 16-byte aligned (4 bytes into a buffer)
-    takes the one-store-per-step backtrack; the 16-byte-block stores of the
+    takes the one-store-per-step backtrack; the 64-byte-block stores of the
     aligned case must give the same entries, and the word before the pointer
     stays untouched."""
     N = 120
@@ -139,10 +139,9 @@ def test_sparse_steps_long_lists(L):
 
 
 def test_sparse_steps_mixed_tiles():
-    """A banded graph whose first 64 rows are dense-ish: the first j tile's
-    neighbour union exceeds the LDS staging capacity (its outputs read H_p
-    per list entry), the others fold from their staged unions; lengths and
-    snapshots must equal the oracle."""
+    """A banded graph whose first 64 rows are dense-ish (neighbour lists of
+    about 200 entries beside lists of 11): waves of one step walk lists of
+    very different lengths; lengths and snapshots must equal the oracle."""
     N, L = 700, 5
     print(f"apsp mixed tiles N={N} L={L}", flush=True)
     rng = np.random.default_rng(77)
@@ -166,13 +165,15 @@ def test_sparse_steps_mixed_tiles():
 
 @pytest.mark.parametrize("N,L,diag", [(160, 11, "zero"), (160, 11, "positive"),
                                       (160, 11, "negzero"), (96, 1, "zero"), (96, 2, "zero"),
-                                      (700, 9, "zero")])
+                                      (700, 9, "zero"), (300, 45, "zero"), (258, 37, "zero")])
 def test_backtrack_stay_gate(N, L, diag):
     """Directed graphs (H_p far from symmetric, so the gate's bound needs
     both H_{L-1} and H_{L-2}) with a band of short edges, whose paths mostly
     stay at a node for many steps. With an all-+0.0 diagonal the backtrack
     skips the scans the gate rules out; a positive or -0.0 diagonal entry
-    turns the gate off. Paths must equal the oracle in every case."""
+    turns the gate off. L = 37 and 45 fill several 16-entry path blocks per
+    (i, j) (N = 258: rows that are not 16-byte multiples stage through
+    registers). Paths must equal the oracle in every case."""
     print(f"apsp stay gate N={N} L={L} diag={diag}", flush=True)
     rng = np.random.default_rng(N + L)
     A = rng.uniform(0.05, 2.0, size=(N, N)).astype(np.float32)
