@@ -287,11 +287,16 @@ int nsx_select_prefix_host(const float *h_key, int n, int m, int *h_idx,
  * p = 0..L-1 from H_0 = A, double-buffered (Jacobi), d_Hi[p] = H_p
  * (L*N*N floats, nullable when d_paths is null), then the backtrack of
  * :70-107 into d_paths[i][j][0..L-1] (N*L*N ints). Diagonal: paths[i][i][*] = i.
- * With d_paths, N <= 92680. */
+ * With d_paths, N <= 92680. Both Hedetniemi entry points enqueue A's
+ * neighbour lists on `stream`, then wait on the host until that work (and
+ * whatever `stream` already held) has produced the sparse/dense flag
+ * (polled from pinned host memory), then enqueue the L steps (and the
+ * backtrack) and return without further waiting. */
 int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H,
                    float *d_Hi, int *d_paths, void *stream);
 
-/* hedetniemiAllShortestPathLengthsGpu (hedetniemi_gpu.cu:109-146): H only. */
+/* hedetniemiAllShortestPathLengthsGpu (hedetniemi_gpu.cu:109-146): H only.
+ * d_H may be d_A itself (the lengths overwrite A). */
 int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H,
                            void *stream);
 
