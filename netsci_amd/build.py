@@ -27,8 +27,11 @@ HOST_SOURCES = ["host_cpu.cpp"]
 PY_SOURCES = ["host/netcalc.cpp", "host/netchem.cpp", "host/py_module.cpp"]
 
 # Packed FP32 instructions stay off: hipcc would otherwise form v_pk_* ops.
+# Kernel arguments (up to 16 SGPRs) are preloaded into SGPRs at wave launch
+# (gfx950), so short kernels do not start with a dependent argument load.
 HIPCC_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-mllvm", "-amdgpu-kernarg-preload-count=16",
                "-Wall", "-Wno-unused-function"]
 GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
              "-pthread", "-Wall"]

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
     const float *__restrict__ Hp, const long long *__restrict__ rows, const int *__restrict__ lst,
     const unsigned *__restrict__ dense, float *__restrict__ out, int N, int nI, int nJ) {
     __shared__ float T[64][TJ + 1];
-    if (dense[0]) return;
+    (void)dense;  // launched only for sparse lists (the host polls the flag)
     const int g = blockIdx.x;
     const int s = g >> 3;
     const int iI = (g & 7) + 8 * (s / nJ);
