@@ -146,14 +146,15 @@ __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ 
 // where nbr(j) = { k : A[j][k] < +inf }: every other k gives +inf or NaN,
 // which the select never takes, so the result equals the dense fold bit for
 // bit (signed zeros and NaN included).
-// Workgroup = a 64 (i) x 64 (j) output tile of 16 waves; lanes own i, each
-// wave takes 4 of the j (16 waves rather than 4 keep enough row reads in
-// flight at small N: 0.77 -> 0.41 ms for 32 steps at N = 1000). For one j
-// the neighbour list is wave-uniform (scalar loads) and every k reads the
-// coalesced row segment Hp[k][i0..i0+63]. Results go
-// through an LDS tile so the output is written as row segments
-// out[i][j0..j0+63]. Workgroups g, g+8, g+16, ... share an XCD; they get the
-// same i-range, so that XCD's L2 keeps the 64-column slice of Hp they read.
+// Workgroup = a (64 NIT) (i) x TJ (j) output tile of 16 waves; lanes own i
+// (NIT columns 64 apart), each wave takes TJ / 16 of the j (16 waves rather
+// than 4 keep enough row reads in flight at small N: 0.77 -> 0.41 ms for 32
+// steps at N = 1000). For one j the neighbour list is wave-uniform (scalar
+// loads) and every k reads the coalesced row segments Hp[k][i0..i0+63] of
+// the wave's NIT tiles. Results go through an LDS tile so the output is
+// written as row segments out[i][j0..j0+TJ). Workgroups g, g+8, g+16, ...
+// share an XCD; they get the same i-range, so that XCD's L2 keeps the
+// column slice of Hp they read.
 // Measured alternatives at N = 4000, L = 132 (32 steps lengths only, this
 // kernel 10.9 ms; all bit-identical): lists padded to 16 entries so no
 // 4-entry tail groups remain 11.5 ms; 16-entry rounds with per-group guards
@@ -164,67 +165,89 @@ __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ 
 // HBM, VALU ~55% busy.
 // TJ = 16 (one j per wave) for small N, where the 64-wide tiles would give
 // each CU only a few waves, each with a chain of four lists to walk.
+// Two i-tiles per wave (kStepTiles): every list entry's scalar loads serve
+// two row reads and the grid has half the waves (one round at N = 1000):
+// N = 1000 lengths 0.374 -> 0.339 ms, N = 4000 10.43 -> 9.66 ms; four tiles
+// need 102 VGPRs and halve the resident waves (0.65 and 12.9 ms).
 constexpr int kStepWaves = 16;  // waves per sparse-step workgroup
+constexpr int kStepTiles = 2;   // 64-column i-tiles per wave
 
-template <int TJ>
+// NIT: 64-column i-tiles per wave; each list entry's scalar loads then
+// serve NIT row reads, and the grid has 1/NIT as many waves.
+template <int TJ, int NIT>
 __global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
     const float *__restrict__ Hp, const long long *__restrict__ rows, const int *__restrict__ lst,
     const unsigned *__restrict__ dense, float *__restrict__ out, int N, int nI, int nJ) {
-    __shared__ float T[64][TJ + 1];
+    __shared__ float T[64 * NIT][TJ + 1];
     (void)dense;  // launched only for sparse lists (the host polls the flag)
     const int g = blockIdx.x;
     const int s = g >> 3;
-    const int iI = (g & 7) + 8 * (s / nJ);
+    const int iI = (g & 7) + 8 * (s / nJ);  // group of NIT i-tiles
     const int jJ = s % nJ;
     if (iI >= nI) return;
-    const int i0 = iI * 64, j0 = jJ * TJ;
+    const int i0 = iI * 64 * NIT, j0 = jJ * TJ;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int i = i0 + lane;
-    const float *col = Hp + (i < N ? i : 0);  // col[k * N] = Hp[k][i]
+    const float *col[NIT];  // col[t][k * N] = Hp[k][i0 + 64 t + lane]
+#pragma unroll
+    for (int t = 0; t < NIT; ++t) {
+        const int i = i0 + 64 * t + lane;
+        col[t] = Hp + (i < N ? i : 0);
+    }
     for (int jj = w; jj < TJ; jj += kStepWaves) {
         const int j = j0 + jj;
-        float c = __builtin_inff();
+        float c[NIT];
+#pragma unroll
+        for (int t = 0; t < NIT; ++t) c[t] = __builtin_inff();
         if (j < N) {
             const long long end = rows[j + 1];
             long long e = rows[j];
-            // 16 entries per round: the list loads and the 16 row reads are
+            // 16 entries per round: the list loads and the row reads are
             // all in flight before the first fold.
             for (; e + 16 <= end; e += 16) {
                 int kq[16];
-                float ka[16], v[16];
+                float ka[16], v[NIT][16];
 #pragma unroll
                 for (int u = 0; u < 16; ++u) {
                     kq[u] = lst[2 * e + 8 * (u >> 2) + (u & 3)];
                     ka[u] = __int_as_float(lst[2 * e + 8 * (u >> 2) + 4 + (u & 3)]);
                 }
 #pragma unroll
-                for (int u = 0; u < 16; ++u) v[u] = col[(long long)kq[u] * N];
+                for (int u = 0; u < 16; ++u)
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const float t = ka[u] + v[u];
-                    c = (t < c) ? t : c;
-                }
+                    for (int t = 0; t < NIT; ++t) v[t][u] = col[t][(long long)kq[u] * N];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int t = 0; t < NIT; ++t) {
+                        const float x = ka[u] + v[t][u];
+                        c[t] = (x < c[t]) ? x : c[t];
+                    }
             }
             for (; e < end; e += 4) {
                 const int4 k4 = *reinterpret_cast<const int4 *>(lst + 2 * e);
                 const float4 a4 = *reinterpret_cast<const float4 *>(lst + 2 * e + 4);
-                const float v0 = col[(long long)k4.x * N], v1 = col[(long long)k4.y * N];
-                const float v2 = col[(long long)k4.z * N], v3 = col[(long long)k4.w * N];
-                float t = a4.x + v0;
-                c = (t < c) ? t : c;
-                t = a4.y + v1;
-                c = (t < c) ? t : c;
-                t = a4.z + v2;
-                c = (t < c) ? t : c;
-                t = a4.w + v3;
-                c = (t < c) ? t : c;
+                const int ks[4] = {k4.x, k4.y, k4.z, k4.w};
+                const float as[4] = {a4.x, a4.y, a4.z, a4.w};
+                float v[NIT][4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int t = 0; t < NIT; ++t) v[t][u] = col[t][(long long)ks[u] * N];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int t = 0; t < NIT; ++t) {
+                        const float x = as[u] + v[t][u];
+                        c[t] = (x < c[t]) ? x : c[t];
+                    }
             }
         }
-        T[lane][jj] = c;
+#pragma unroll
+        for (int t = 0; t < NIT; ++t) T[64 * t + lane][jj] = c[t];
     }
     __syncthreads();
-    for (int r = w; r < 64; r += kStepWaves) {
+    for (int r = w; r < 64 * NIT; r += kStepWaves) {
         const int ii = i0 + r, j = j0 + lane;
         if (lane < TJ && ii < N && j < N) out[(long long)ii * N + j] = T[r][lane];
     }
@@ -738,14 +761,18 @@ int minplus_step(const float *A, const float *Hp, float *out, int N, const Lists
         const int nI = (N + 63) / 64;
         // 64-wide j tiles when they give every CU at least four workgroups.
         if ((long long)nI * nI >= 4LL * cu_count()) {
-            const int groups = 8 * ((nI + 7) / 8) * nI;
-            hipLaunchKernelGGL(minplus_sparse_kernel<64>, dim3(groups), dim3(kStepWaves * 64), 0,
-                               s, Hp, la.rows, la.lst, la.dense, out, N, nI, nI);
+            const int nIg = (nI + kStepTiles - 1) / kStepTiles;
+            const int groups = 8 * ((nIg + 7) / 8) * nI;
+            hipLaunchKernelGGL((minplus_sparse_kernel<64, kStepTiles>), dim3(groups),
+                               dim3(kStepWaves * 64), 0, s, Hp, la.rows, la.lst, la.dense, out, N,
+                               nIg, nI);
         } else {
             const int nJ = (N + 15) / 16;
-            const int groups = 8 * ((nI + 7) / 8) * nJ;
-            hipLaunchKernelGGL(minplus_sparse_kernel<16>, dim3(groups), dim3(kStepWaves * 64), 0,
-                               s, Hp, la.rows, la.lst, la.dense, out, N, nI, nJ);
+            const int nIg = (nI + kStepTiles - 1) / kStepTiles;
+            const int groups = 8 * ((nIg + 7) / 8) * nJ;
+            hipLaunchKernelGGL((minplus_sparse_kernel<16, kStepTiles>), dim3(groups),
+                               dim3(kStepWaves * 64), 0, s, Hp, la.rows, la.lst, la.dense, out, N,
+                               nIg, nJ);
         }
         NSX_HIP(hipGetLastError(), "sparse min-plus launch");
         return kOk;
