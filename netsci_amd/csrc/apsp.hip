@@ -168,7 +168,10 @@ __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ 
 // Two i-tiles per wave (kStepTiles): every list entry's scalar loads serve
 // two row reads and the grid has half the waves (one round at N = 1000):
 // N = 1000 lengths 0.374 -> 0.339 ms, N = 4000 10.43 -> 9.66 ms; four tiles
-// need 102 VGPRs and halve the resident waves (0.65 and 12.9 ms).
+// need 102 VGPRs and halve the resident waves (0.65 and 12.9 ms); two
+// adjacent columns per lane by one 8-byte load, 0.41 and 14.3 ms; the tail
+// groups as one more 16-entry round that repeats its last group, 0.37 and
+// 10.9 ms.
 constexpr int kStepWaves = 16;  // waves per sparse-step workgroup
 constexpr int kStepTiles = 2;   // 64-column i-tiles per wave
 
