@@ -202,6 +202,70 @@ def test_lengths_only_matches():
     np.testing.assert_array_equal(dH.numpy().view(np.uint32), Ho.view(np.uint32))
 
 
+@pytest.mark.parametrize("N,L", [(5200, 5), (8200, 3)])
+def test_large_n_backtrack_variants_sampled(N, L):
+    """N = 5200 takes the backtrack whose list starts stay in global memory
+    (its row buffers and bound table fill the workgroup's LDS) and N = 8200
+    the one that reads snapshot rows from L2 instead of LDS. The dense oracle
+    is too slow here, so 400 seeded (i, j) are checked bit for bit against
+    the recurrence itself: every sampled H_{p+1}[i][j] refolded from the
+    GPU's own H_p (ascending k, strict `<`), and every sampled paths row
+    re-derived by the first-hit rule (hedetniemi_gpu.cu:86-104) from the
+    GPU's own snapshots."""
+    print(f"apsp large-N backtrack N={N} L={L}", flush=True)
+    rng = np.random.default_rng(N)
+    band = 3
+    A = np.full((N, N), np.inf, np.float32)
+    for o in range(-band, band + 1):
+        if o:
+            r = np.arange(max(0, -o), N - max(0, o))  # rows whose column r + o exists
+            A[r, r + o] = rng.uniform(0.05, 2.0, len(r)).astype(np.float32)
+    far = rng.integers(0, N, size=(4 * N, 2))
+    A[far[:, 0], far[:, 1]] = rng.uniform(0.5, 3.0, len(far)).astype(np.float32)
+    np.fill_diagonal(A, 0.0)
+    tol = np.float32(1e-4)
+    dA = abi.DeviceArray.from_numpy(A)
+    dH = abi.DeviceArray((N, N), np.float32)
+    dHi = abi.DeviceArray((L, N, N), np.float32)
+    dP = abi.DeviceArray((N * L * N,), np.int32)
+    abi.hedetniemi(dA, N, L, float(tol), dH, dHi, dP)
+    H, Hi = dH.numpy(), dHi.numpy()
+    np.testing.assert_array_equal(Hi[0].view(np.uint32), A.view(np.uint32))
+    cand_row = [np.nonzero(A[j] < np.inf)[0] for j in range(N)]     # step candidates of j
+    cand_col = [np.nonzero(A[:, k] < np.inf)[0] for k in range(N)]  # backtrack candidates of k
+    ij = rng.integers(0, N, size=(400, 2))
+    row = np.empty(L, np.int32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        for i, j in ij:
+            i, j = int(i), int(j)
+            # Steps: H_{p+1}[i][j] = first minimum over k of fl(A[j][k] + H_p[k][i]).
+            for p in range(L):
+                nxt = H if p + 1 == L else Hi[p + 1]
+                c = np.float32(np.inf)
+                for k in cand_row[j]:
+                    t = np.float32(A[j, k] + Hi[p][k, i])
+                    if t < c:
+                        c = t
+                assert nxt[i, j].view(np.uint32) == c.view(np.uint32), (i, j, p)
+            # Backtrack.
+            off = ((i * L * N) + j * L) * 4
+            abi.check(abi.lib().nsx_memcpy_d2h(row.ctypes.data, ctypes.c_void_p(dP.ptr.value + off),
+                                                L * 4), "paths row")
+            if i == j:
+                assert (row == i).all()
+                continue
+            ln = np.float32(H[i, j])
+            k = j
+            for p in range(L - 1, -1, -1):
+                hrow = Hi[p][i]
+                for q in cand_col[k]:
+                    if q != j and np.float32(np.float32(hrow[q] + A[q, k]) - ln) < tol:
+                        ln = np.float32(ln - A[q, k])
+                        k = int(q)
+                        break
+                assert row[p] == k, (i, j, p, row.tolist())
+
+
 @pytest.mark.parametrize("L,in_place", [(0, False), (1, False), (4, False), (3, True),
                                          (4, True)])
 def test_lengths_first_step_reads_A(L, in_place):
