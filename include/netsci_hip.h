@@ -10,6 +10,10 @@
  *   - `stream` is a hipStream_t passed as void*; NULL is the default stream.
  *     Entry points that take a stream are stream-ordered and return before
  *     the work finishes unless stated otherwise.
+ *   - The compute entry points share per-process scratch (grow-only device
+ *     workspaces, the table buffer, a pinned flag block): issue them from one
+ *     host thread at a time, and do not overlap calls on different streams;
+ *     work queued on one stream runs in order.
  *   - X holds one node per row. Within a row, coordinate dimension di of
  *     frame f sits at X[row*ld + di*dstride + f] — the reference's layout
  *     (netchem/src/network.cpp:65-69, mutual_information_cpu.cpp:27-45

@@ -50,6 +50,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "../../include/netsci_hip.h"
@@ -797,11 +798,12 @@ int minplus_step(const float *A, const float *Hp, float *out, int N, const Lists
 // stream synchronisation; if the stream completes without them (or the
 // poll budget runs out) they are read back with a copy.
 int step_setup(const float *A, int N, hipStream_t s, Lists &la) {
-    static unsigned seq = 0;
+    static std::atomic<unsigned> seq{0};
     unsigned *hostf = nullptr, *devf = nullptr;
     int rc = host_flags(&hostf, &devf);
     if (rc) return rc;
-    const unsigned my = ++seq;
+    unsigned my = seq.fetch_add(1u) + 1u;
+    if (my == 0u) my = seq.fetch_add(1u) + 1u;  // 0 is the block's initial value
     rc = build_lists(A, N, kWsApspRowsA, kWsApspCandA, true, false, devf, my, s, la);
     if (rc) return rc;
     volatile unsigned *hf = hostf;
