@@ -676,7 +676,6 @@ __global__ __launch_bounds__(kWave *build_block_waves(32 * W), 3) void list_buil
     constexpr int kRows = kBW / W;
     constexpr unsigned kPad = 0xffffffffu;  // above every packed finite key
     const int n = a.n;
-    const int lane = threadIdx.x & (kWave - 1);
     const int wib = threadIdx.x / kWave;
     const int rg = wib / W, wl = wib - rg * W;  // row in the workgroup, block in the row
     const int ebase = wl * 2048;
