@@ -133,11 +133,19 @@ int nsx_gc_from_mi_host(const float *h_mi, long long P, int d, float *h_gc) {
         nsx::set_error("nsx_gc_from_mi_host: invalid arguments");
         return 1;
     }
-    for (long long p = 0; p < P; ++p) {
-        const float mi = h_mi[p];
-        if (mi <= 0.0) h_gc[p] = 0.0f;
-        else h_gc[p] = (float)std::sqrt(1.0 - (float)std::exp(-(2.0 / (float)d) * mi));
-    }
+    // Chunks of 32768 values on up to 16 threads (each value on its own, so
+    // the split does not change a bit); short inputs stay on this thread.
+    constexpr long long kChunk = 32768;
+    const long long chunks = (P + kChunk - 1) / kChunk;
+    const int threads = P < 2 * kChunk ? 1 : std::min(pick_threads(0), 16);
+    parallel_for(chunks, threads, [&](long long c) {
+        const long long end = std::min(P, (c + 1) * kChunk);
+        for (long long p = c * kChunk; p < end; ++p) {
+            const float mi = h_mi[p];
+            if (mi <= 0.0) h_gc[p] = 0.0f;
+            else h_gc[p] = (float)std::sqrt(1.0 - (float)std::exp(-(2.0 / (float)d) * mi));
+        }
+    });
     return 0;
 }
 

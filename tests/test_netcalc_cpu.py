@@ -201,3 +201,22 @@ def test_corr_to_adj_cpu():
     A = cuarray.FloatCuArray()
     netcalc.correlationToAdjacency(A=A, C=_fa(C), n=9, platform=netcalc.CPU_PLATFORM)
     np.testing.assert_array_equal(A.toNumpy2D(), oracle.corr_to_adj(C))
+
+
+def test_gc_transform_threaded_chunks_equal_oracle():
+    """The host GC transform (generalized_correlation_cpu.cpp:8-27) runs in
+    chunks of 32,768 values on several threads for long inputs; every value
+    must equal the oracle's, at the chunk boundaries too, with MI <= 0
+    (including -0.0) mapped to 0."""
+    rng = np.random.default_rng(5)
+    P = 3 * 32768 + 123
+    mi = rng.uniform(-0.2, 2.0, P).astype(np.float32)
+    mi[::997] = 0.0
+    mi[5] = -0.0
+    idx = np.unique(np.concatenate([np.arange(0, P, 331), [0, 5, 32767, 32768, 65535, 65536,
+                                                            98303, 98304, P - 1]]))
+    for d in (1, 2, 3):
+        gc = abi.gc_from_mi(mi, d)
+        want = np.array([oracle.gc_from_mi(float(mi[p]), d) for p in idx], np.float32)
+        np.testing.assert_array_equal(gc[idx].view(np.uint32), want.view(np.uint32))
+        np.testing.assert_array_equal(abi.gc_from_mi(mi[:1000], d), gc[:1000])
