@@ -464,11 +464,21 @@ __global__ void set_slots_kernel(const int *node_of, int count, int base, int *r
 // (mutual_information_cpu.cpp:110-118). A wave owns 64 consecutive pairs,
 // one lane each: tiles of 64 pairs x 64 frames are read row by row
 // (coalesced), turned into psi terms and transposed through LDS, and every
-// lane adds its own pair's terms in frame order.
+// lane adds its own pair's terms in frame order. kPsiLds: the psi table
+// (n + 1 <= 2049 entries) is staged in LDS, so its two lookups per frame are
+// LDS reads instead of scattered global loads (C1: 2.08 -> 1.33 ms).
 constexpr int kSumWaves = 4;
+constexpr int kSumPsiLds = 2048;  // psi entries staged in LDS (n + 1 <= this)
+template <bool kPsiLds>
 __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a) {
     __shared__ float tile[kSumWaves][64][65];
+    __shared__ float psi_lds[kPsiLds ? kSumPsiLds + 1 : 1];
     if (a.abort && *a.abort) return;
+    if (kPsiLds) {
+        for (int t = threadIdx.x; t <= a.n; t += blockDim.x) psi_lds[t] = a.psi[t];
+        __syncthreads();
+    }
+    const float *psi = kPsiLds ? psi_lds : a.psi;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x / 64;
     const long long p0 = ((long long)blockIdx.x * kSumWaves + w) * 64;
@@ -501,7 +511,7 @@ __global__ __launch_bounds__(64 * kSumWaves) void psi_sum_kernel(ksgw::SumArgs a
                     float term = 0.0f;
                     if (f < a.n) {
                         const int nx = (int)(v[u] & 0xffffu), ny = (int)(v[u] >> 16);
-                        term = a.psi[nx] + a.psi[ny];
+                        term = psi[nx] + psi[ny];
                         if (a.counts_out) {
                             const long long at = pr * a.n + f;
                             a.counts_out[at * 2 + 0] = nx;
@@ -1079,8 +1089,12 @@ int list_enqueue(const Problem &pb, const ListLayout &L, const WalkShape &ws, co
                 sa.pbase = p0;
                 sa.abort = abort_flag;
                 const long long sum_blocks = (Pc + 64 * kSumWaves - 1) / (64 * kSumWaves);
-                hipLaunchKernelGGL(psi_sum_kernel, dim3((unsigned)sum_blocks),
-                                   dim3(64 * kSumWaves), 0, stream, sa);
+                if (n + 1 <= kSumPsiLds + 1)
+                    hipLaunchKernelGGL(psi_sum_kernel<true>, dim3((unsigned)sum_blocks),
+                                       dim3(64 * kSumWaves), 0, stream, sa);
+                else
+                    hipLaunchKernelGGL(psi_sum_kernel<false>, dim3((unsigned)sum_blocks),
+                                       dim3(64 * kSumWaves), 0, stream, sa);
                 NSX_HIP(hipGetLastError(), "psi sum launch");
                 if (e3) (void)hipEventRecord(e3, stream);
                 if (pm) {
