@@ -471,8 +471,8 @@ __device__ __forceinline__ int first_pass(const int4 q, const float4 a, const fl
 // one dword store per step.
 // kGroups: list groups whose loads are issued together before their tests
 // (loads past the list's end are skipped); the tests keep the slot order.
-// kGate (with kLdsRow; active when every A[k][k] is +0.0, dense[1]): the
-// stay gate. With a zero diagonal, k = j is a candidate of every Jacobi
+// kGate (active when every A[k][k] is +0.0, dense[1]; instantiated with
+// kLdsRow, the bound also reads global rows without it): the stay gate. With a zero diagonal, k = j is a candidate of every Jacobi
 // fold with weight 0, so H_{p+1}[i][j] <= H_p[j][i] and H_{p+2} <= H_p entry
 // by entry (entries of H_0 = A that are NaN pass no test and need no
 // bound): LB[q] = min(Hi[L-1][i][q], Hi[L-2][i][q]) is below Hi[p][i][q] for
@@ -485,7 +485,7 @@ __device__ __forceinline__ int first_pass(const int4 q, const float4 a, const fl
 // self[k]; otherwise it scans cand(k) from the start. Most steps of a path
 // that has reached its last hop count are such stays, so the gate replaces
 // their scan with two LDS reads. bound[] is computed per workgroup (one i)
-// into LDS after the row buffers and span table.
+// into LDS after the row buffers and the span table (those present).
 template <bool kLdsRow, bool kLdsSpan, bool kQuad, int kGroups, bool kGate>
 __global__ __launch_bounds__(kBtThreads) __attribute__((amdgpu_waves_per_eu(8))) void
 backtrack_sparse_kernel(
