@@ -277,7 +277,13 @@ __global__ void transpose_kernel(const float *__restrict__ in, float *__restrict
 // Mean candidate-list length above which the wave-per-(i, j) kernel runs.
 constexpr long long kDenseDegree = 128;
 
-__device__ __forceinline__ bool is_candidate(float a) { return a < __builtin_inff(); }
+// a < +inf (finite or -inf; NaN and +inf are not), tested on the bits. As
+// a float compare handed straight to a ballot, `a < +inf` (fcmp one) has
+// been emitted as the unordered v_cmp_neq_f32, which takes NaN.
+__device__ __forceinline__ bool is_candidate(float a) {
+    const unsigned b = __float_as_uint(a);
+    return (b & 0x7fffffffu) < 0x7f800000u || b == 0xff800000u;
+}
 
 // Candidate counts: one wave per row k of M stores the list length rounded
 // up to a multiple of 4, and in negz[k] whether the row holds a -0.0 (the
@@ -291,11 +297,19 @@ __global__ __launch_bounds__(256) void cand_count_kernel(const float *__restrict
     const float *row = M + (long long)k * N;
     long long cnt = 0;
     bool nz = false;
-    for (int q0 = 0; q0 < N; q0 += 64) {
-        const int q = q0 + lane;
-        const float a = q < N ? row[q] : 0.0f;
-        cnt += __popcll(__ballot(q < N && is_candidate(a)));
-        nz |= __float_as_uint(a) == 0x80000000u;
+    // Four 64-column chunks' loads in flight before their ballots.
+    for (int q0 = 0; q0 < N; q0 += 256) {
+        float a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = q0 + 64 * u + lane;
+            a[u] = q < N ? row[q] : __builtin_inff();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            cnt += __popcll(__ballot(is_candidate(a[u])));
+            nz |= __float_as_uint(a[u]) == 0x80000000u;
+        }
     }
     const bool any_nz = __ballot(nz) != 0ull;
     if (lane == 0) {
@@ -380,17 +394,28 @@ __global__ __launch_bounds__(256) void cand_fill_kernel(const float *__restrict_
     const long long end = rows[k + 1];
     long long at = rows[k];
     long long own = -1;  // index of the entry q = k (wave-uniform)
-    for (int q0 = 0; q0 < N; q0 += 64) {
-        const int q = q0 + lane;
-        const float a = q < N ? row[q] : __builtin_inff();
-        const bool ok = q < N && is_candidate(a);
-        const unsigned long long m = __ballot(ok);
-        const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (ok) put_entry(lst, at + rank, q, a);
-        if (k >= q0 && k < q0 + 64 && ((m >> (k - q0)) & 1ull))
-            own = at + __popcll(m & ((1ull << (k - q0)) - 1ull));
-        at += __popcll(m);
+    // Four 64-column chunks' loads in flight before their ballots.
+    for (int c0 = 0; c0 < N; c0 += 256) {
+        float av[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q = c0 + 64 * u + lane;
+            av[u] = q < N ? row[q] : __builtin_inff();
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int q0 = c0 + 64 * u;
+            const int q = q0 + lane;
+            const float a = av[u];
+            const bool ok = is_candidate(a);  // +inf past N
+            const unsigned long long m = __ballot(ok);
+            const int rank = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (ok) put_entry(lst, at + rank, q, a);
+            if (k >= q0 && k < q0 + 64 && ((m >> (k - q0)) & 1ull))
+                own = at + __popcll(m & ((1ull << (k - q0)) - 1ull));
+            at += __popcll(m);
+        }
     }
     if (self && lane == 0) {
         self[k] = (int)own;
