@@ -272,7 +272,8 @@ int nsx_screen_select(const float *d_pred, long long P, float threshold,
 
 /* ---- host (CPU platform) implementations -------------------------------
  * The CPU side of the `platform` flag (platform.h:9-10). Same results as the
- * GPU path; threads over pairs. */
+ * GPU path; threads over pairs (threads <= 0: OMP_NUM_THREADS when set,
+ * else every hardware thread; the same for the other CPU-platform calls). */
 int nsx_cpu_ksg_mi(const float *h_X, long long ld, long long dstride, int d,
                    int n, const int *h_ab, long long P, int k,
                    float *h_mi_out, int threads);

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -25,8 +26,14 @@ void psi_table(float *psi, int n);
 
 namespace {
 
+// Worker count: the caller's, else OMP_NUM_THREADS when set (the usual way
+// a job states its CPU share), else every hardware thread.
 int pick_threads(int threads) {
     if (threads > 0) return threads;
+    if (const char *env = std::getenv("OMP_NUM_THREADS")) {
+        const int t = std::atoi(env);
+        if (t > 0) return t;
+    }
     unsigned hc = std::thread::hardware_concurrency();
     return hc ? (int)hc : 1;
 }
