@@ -173,12 +173,20 @@ __global__ __launch_bounds__(256) void minplus_kernel(const float *__restrict__ 
 // adjacent columns per lane by one 8-byte load, 0.41 and 14.3 ms; the tail
 // groups as one more 16-entry round that repeats its last group, 0.37 and
 // 10.9 ms.
+// Write-out: with TJ = 16 a wave stores four 16-column row segments per
+// instruction instead of one (N = 1000 lengths 0.343 -> 0.312 ms). kStream
+// makes those stores non-temporal, so the step leaves no dirty output in
+// L2 for the end-of-kernel write-back: 0.312 -> 0.292 ms for a lengths
+// call, whose outputs are read only by the next step. The path pass keeps
+// plain stores (its snapshots feed the backtrack: 0.558 ms with paths
+// against 0.581 non-temporal), and so does TJ = 64 (N = 4000 lengths 9.61
+// ms plain, 10.95 non-temporal).
 constexpr int kStepWaves = 16;  // waves per sparse-step workgroup
 constexpr int kStepTiles = 2;   // 64-column i-tiles per wave
 
 // NIT: 64-column i-tiles per wave; each list entry's scalar loads then
 // serve NIT row reads, and the grid has 1/NIT as many waves.
-template <int TJ, int NIT>
+template <int TJ, int NIT, bool kStream>
 __global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
     const float *__restrict__ Hp, const long long *__restrict__ rows, const int *__restrict__ lst,
     const unsigned *__restrict__ dense, float *__restrict__ out, int N, int nI, int nJ) {
@@ -251,9 +259,14 @@ __global__ __launch_bounds__(kStepWaves * 64) void minplus_sparse_kernel(
         for (int t = 0; t < NIT; ++t) T[64 * t + lane][jj] = c[t];
     }
     __syncthreads();
-    for (int r = w; r < 64 * NIT; r += kStepWaves) {
-        const int ii = i0 + r, j = j0 + lane;
-        if (lane < TJ && ii < N && j < N) out[(long long)ii * N + j] = T[r][lane];
+    constexpr int kRowsPerStore = 64 / TJ;
+    const int rr = lane / TJ, cc = lane % TJ;
+    for (int r = w * kRowsPerStore + rr; r < 64 * NIT; r += kStepWaves * kRowsPerStore) {
+        const int ii = i0 + r, j = j0 + cc;
+        if (ii < N && j < N) {
+            if constexpr (kStream) __builtin_nontemporal_store(T[r][cc], out + (long long)ii * N + j);
+            else out[(long long)ii * N + j] = T[r][cc];
+        }
     }
 }
 
@@ -784,24 +797,30 @@ int build_lists(const float *M, int N, int slot_rows, int slot_cand, bool allow_
 
 // One Jacobi step out = step(Hp): the sparse or the dense kernel, as the
 // dense flag of A's neighbour lists (read back once per call) says.
+// `stream_out`: out is read only by the next step (lengths-only calls).
 int minplus_step(const float *A, const float *Hp, float *out, int N, const Lists &la,
-                 hipStream_t s) {
+                 bool stream_out, hipStream_t s) {
     if (!la.host_dense) {
         const int nI = (N + 63) / 64;
         // 64-wide j tiles when they give every CU at least four workgroups.
         if ((long long)nI * nI >= 4LL * cu_count()) {
             const int nIg = (nI + kStepTiles - 1) / kStepTiles;
             const int groups = 8 * ((nIg + 7) / 8) * nI;
-            hipLaunchKernelGGL((minplus_sparse_kernel<64, kStepTiles>), dim3(groups),
+            hipLaunchKernelGGL((minplus_sparse_kernel<64, kStepTiles, false>), dim3(groups),
                                dim3(kStepWaves * 64), 0, s, Hp, la.rows, la.lst, la.dense, out, N,
                                nIg, nI);
         } else {
             const int nJ = (N + 15) / 16;
             const int nIg = (nI + kStepTiles - 1) / kStepTiles;
             const int groups = 8 * ((nIg + 7) / 8) * nJ;
-            hipLaunchKernelGGL((minplus_sparse_kernel<16, kStepTiles>), dim3(groups),
-                               dim3(kStepWaves * 64), 0, s, Hp, la.rows, la.lst, la.dense, out, N,
-                               nIg, nJ);
+            if (stream_out)
+                hipLaunchKernelGGL((minplus_sparse_kernel<16, kStepTiles, true>), dim3(groups),
+                                   dim3(kStepWaves * 64), 0, s, Hp, la.rows, la.lst, la.dense,
+                                   out, N, nIg, nJ);
+            else
+                hipLaunchKernelGGL((minplus_sparse_kernel<16, kStepTiles, false>), dim3(groups),
+                                   dim3(kStepWaves * 64), 0, s, Hp, la.rows, la.lst, la.dense,
+                                   out, N, nIg, nJ);
         }
         NSX_HIP(hipGetLastError(), "sparse min-plus launch");
         return kOk;
@@ -885,7 +904,7 @@ int nsx_hedetniemi_lengths(const float *d_A, int N, int L, float *d_H, void *str
     }
     for (int p = 0; p < L; ++p) {
         float *out = ((L - 1 - p) % 2 == 0) ? d_H : tmp;
-        rc = minplus_step(d_A, in, out, N, la, s);
+        rc = minplus_step(d_A, in, out, N, la, true, s);
         if (rc) return rc;
         in = out;
     }
@@ -911,7 +930,7 @@ int nsx_hedetniemi(const float *d_A, int N, int L, float tol, float *d_H, float 
     NSX_HIP(hipMemcpyAsync(d_Hi, d_A, bytes, hipMemcpyDeviceToDevice, s), "Hi[0] = A");
     for (int p = 0; p < L; ++p) {
         float *out = (p + 1 < L) ? d_Hi + (size_t)(p + 1) * NN : d_H;
-        rc = minplus_step(d_A, d_Hi + (size_t)p * NN, out, N, la, s);
+        rc = minplus_step(d_A, d_Hi + (size_t)p * NN, out, N, la, false, s);
         if (rc) return rc;
     }
     if (!d_paths) return kOk;
